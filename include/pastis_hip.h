@@ -1,0 +1,209 @@
+/*
+ * pastis_hip.h -- C ABI of the MI355X batched pairwise protein aligner.
+ *
+ * This is the device boundary behind the reference's aligner operator
+ * pastis::PWAlign (inc/align/PWAlign.hpp:20-82).  Plain C types only: no
+ * C++ exceptions, no torch or HIP types cross it.  Every call returns an
+ * int status (PA_OK == 0); pa_strerror() names it and pa_last_error()
+ * gives the context's last message.
+ *
+ * Sequence conventions follow the reference call sites:
+ *   - the ROW sequence set is rseqs_ (SeqanFullAligner.hpp:27), indexed by
+ *     the tuple's lr + bl_roffset; it is SeqAn's vertical sequence (seqsr);
+ *   - the COLUMN sequence set is cseqs_ (SeqanFullAligner.hpp:28), indexed
+ *     by lc + bl_coffset; it is SeqAn's horizontal sequence (seqsc).
+ *   - residues are codes 0..24 in the order "ARNDCQEGHILKMFPSTWYVBZX*J"
+ *     (src/Alphabet.cpp:26), or ASCII letters when loaded with
+ *     PA_ENC_ASCII (case-insensitive, unknown letters become X);
+ *   - a sequence holds at most 65535 residues (lengths are ushort in
+ *     inc/FastaData.hpp:109-118).
+ * Gap model of the DP kernels: a gap of length L costs
+ *   gap_open + (L-1)*gap_ext   (both <= 0),
+ * i.e. SeqAn's scoreGapOpen/scoreGapExtend.  pa_config_for_cli() maps the
+ * reference's CLI values (-g, -e) onto that model (see PA_GAP_AS_CALLED).
+ */
+#ifndef PASTIS_HIP_H
+#define PASTIS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define PA_OK 0
+#define PA_ERR_ARG 1        /* bad argument (null pointer, bad enum, ...) */
+#define PA_ERR_HIP 2        /* HIP runtime error */
+#define PA_ERR_NOMEM 3      /* device or host allocation failed */
+#define PA_ERR_SEQ_LEN 4    /* a sequence is longer than 65535 */
+#define PA_ERR_RESIDUE 5    /* a residue code is outside 0..24 */
+#define PA_ERR_INDEX 6      /* a pair names a sequence that is not loaded */
+#define PA_ERR_SEED 7       /* a seed does not fit inside its sequences */
+#define PA_ERR_NOSEQ 8      /* sequences were not loaded */
+#define PA_ERR_NODEV 9      /* no usable gfx950 device */
+#define PA_ERR_LIMIT 10     /* a size limit of the entry point is exceeded */
+
+/* ---- enums ----------------------------------------------------------- */
+enum pa_mode { PA_MODE_SW = 0,     /* --sfa: full local Smith-Waterman */
+               PA_MODE_XDROP = 1   /* --sxa: seed-and-extend X-drop    */ };
+enum pa_side { PA_SIDE_ROW = 0, PA_SIDE_COL = 1, PA_SIDE_BOTH = 2 };
+enum pa_enc { PA_ENC_CODES = 0, PA_ENC_ASCII = 1 };
+enum pa_gap_model {
+    /* SeqAn's Score(gapExtend, gapOpen) constructor receives the reference's
+     * (gap_open, gap_ext) positionally (SeqanFullAligner.hpp:35-37), so the
+     * CLI's -g is SeqAn's extension and -e its opening. */
+    PA_GAP_AS_CALLED = 0,
+    /* the CLI's documented meaning: -g opens, -e extends */
+    PA_GAP_AS_DOCUMENTED = 1
+};
+
+/* result flags */
+#define PA_FLAG_PASS 0x1     /* coverage and identity predicate held */
+#define PA_FLAG_RERUN32 0x2  /* the int16 DP saturated; int32 re-run used */
+#define PA_FLAG_EMPTY 0x4    /* local score 0: empty alignment */
+
+typedef struct pa_config {
+    int32_t mode;         /* enum pa_mode */
+    int32_t gap_open;     /* DP gap model, see top of file */
+    int32_t gap_ext;
+    int32_t xdrop;        /* X of the X-drop extension (--sxa X) */
+    int32_t xdrop_gap;    /* linear gap of the extension (SeqAn scoreGap) */
+    int32_t seed_len;     /* k-mer length used as seed length (-k) */
+    int32_t seed_cnt;     /* seeds tried per pair (--sc), 1 or 2 */
+    int32_t ani_thr;      /* aln_ani_thr, src/main.cpp:110 */
+    double cov_thr;       /* aln_cov_thr, src/main.cpp:109 */
+    int32_t device;       /* HIP device ordinal */
+    int32_t flags;        /* reserved, 0 */
+    const int8_t *matrix; /* 32x32 row-major int8 scores, NULL = BLOSUM62 */
+} pa_config;
+
+/* One candidate pair.  r, c are indices into the loaded row / column sets
+ * (the tuple's lr + bl_roffset and lc + bl_coffset).  count, kscore and
+ * seed mirror CommonKmerLoc (inc/Types.hpp:163-215): seed[0..1] =
+ * first.(row offset, col offset), seed[2..3] = second.(row, col). */
+typedef struct pa_pair {
+    uint32_t r, c;
+    uint16_t count;
+    int16_t kscore;
+    uint16_t seed[4];
+} pa_pair;
+
+/* Full per-pair result of pa_align. */
+typedef struct pa_result {
+    int32_t score;           /* local score (SW) / global infix score (X-drop) */
+    uint16_t r_beg, r_end;   /* aligned range in the row sequence [beg, end) */
+    uint16_t c_beg, c_end;   /* aligned range in the column sequence */
+    uint32_t matches;        /* AlignmentStats::numMatches */
+    uint32_t cols;           /* AlignmentStats::alignmentLength */
+    uint32_t gap_opens;      /* AlignmentStats::numGapOpens */
+    float identity;          /* AlignmentStats::alignmentIdentity */
+    float score_aln;         /* identity/100 if PA_FLAG_PASS else 0 */
+    uint16_t flags;          /* PA_FLAG_* */
+    uint16_t seed;           /* winning seed (X-drop mode) */
+} pa_result;
+
+/* Score and end cell of a local alignment (no traceback). */
+typedef struct pa_score {
+    int32_t score;
+    uint16_t r_end, c_end;   /* exclusive ends; 0,0 when score is 0 */
+} pa_score;
+
+/* Seed extension of one seed. */
+typedef struct pa_extent {
+    uint16_t r_beg, r_end, c_beg, c_end;
+} pa_extent;
+
+/* Counters and timers of the calls made on a context so far. */
+typedef struct pa_stats {
+    uint64_t pairs;          /* pairs aligned */
+    uint64_t dp_cells;       /* DP cells of the SW / global kernels */
+    uint64_t xdrop_cells;    /* cells evaluated by the X-drop kernel */
+    uint64_t rerun_pairs;    /* tasks re-run in int32 */
+    double ms_sw;            /* device time of the SW kernels */
+    double ms_xdrop;         /* device time of the X-drop kernels */
+    double ms_nw;            /* device time of the global-alignment kernels */
+    double ms_trace;         /* device time of the traceback kernels */
+    double ms_total;         /* device time of whole calls (event to event) */
+    uint64_t launches;       /* kernel launches */
+} pa_stats;
+
+/* ---- lifetime -------------------------------------------------------- */
+
+/* Replaces the aligner constructors SeqanFullAligner(gap_open, gap_ext)
+ * (inc/align/SeqanFullAligner.hpp:35-37) and SeqanXdropAligner(gap_open,
+ * gap_ext, seed_len, xdrop, seed_cnt) (inc/align/SeqanXdropAligner.hpp:38-42),
+ * called from src/sim_search.cpp:119-124. */
+int pa_create(const pa_config *cfg, struct pa_ctx **out);
+
+/* Fills cfg from the reference's CLI values (-g, -e, -k, --sxa X, --sc) and a
+ * gap model (enum pa_gap_model); mode is PA_MODE_SW when xdrop <= 0. */
+int pa_config_for_cli(pa_config *cfg, int gap_open_cli, int gap_ext_cli,
+                      int klength, int xdrop, int seed_count, int gap_model);
+
+/* Replaces ~SeqanFullAligner / ~SeqanXdropAligner. */
+void pa_destroy(struct pa_ctx *ctx);
+
+const char *pa_strerror(int status);
+const char *pa_last_error(struct pa_ctx *ctx);
+
+/* ---- sequences ------------------------------------------------------- */
+
+/* Replaces construct_seqs / construct_seqs_bl
+ * (src/align/SeqanFullAligner.cpp:28-168, SeqanXdropAligner.cpp:32-185):
+ * uploads one sequence set once.  residues[offsets[i] .. offsets[i+1]) is
+ * sequence i; offsets has nseq+1 entries.  PA_SIDE_BOTH makes the column set
+ * the same as the row set (the diagonal-block case, SeqanFullAligner.cpp:82-87). */
+int pa_load_seqs(struct pa_ctx *ctx, int side, const uint8_t *residues,
+                 const uint64_t *offsets, uint64_t nseq, int encoding);
+
+/* ---- the operator ---------------------------------------------------- */
+
+/* Replaces PWAlign::aln_batch for both tuple types
+ * (inc/align/PWAlign.hpp:45-63; SeqanFullAligner.cpp:173-266 in PA_MODE_SW,
+ * SeqanXdropAligner.cpp:190-361 in PA_MODE_XDROP).  Host pointers; n == 0 is
+ * legal (the beg == end batch of src/DistPWRunner.cpp:85-86,118-131).
+ * Synchronous. */
+int pa_align(struct pa_ctx *ctx, const pa_pair *pairs, uint64_t n,
+             pa_result *out);
+
+/* The same on device-resident pairs and results (pointers from
+ * pa_device_alloc or hipMalloc on the context's device); stream may be NULL
+ * for the context's own stream.  Returns after the work is queued; the
+ * results are complete when the stream is. */
+int pa_align_dev(struct pa_ctx *ctx, const pa_pair *d_pairs, uint64_t n,
+                 pa_result *d_out, void *stream);
+
+/* Score and end cell only: the localAlignment call of
+ * SeqanFullAligner.cpp:231 without traceback.  Host and device variants. */
+int pa_sw_scores(struct pa_ctx *ctx, const pa_pair *pairs, uint64_t n,
+                 pa_score *out);
+int pa_sw_scores_dev(struct pa_ctx *ctx, const pa_pair *d_pairs, uint64_t n,
+                     pa_score *d_out, void *stream);
+
+/* Seed extension only: extendSeed(..., GappedXDrop) of
+ * SeqanXdropAligner.cpp:279-289 for seed `seed` (0 or 1) of every pair. */
+int pa_xdrop_extents(struct pa_ctx *ctx, const pa_pair *pairs, uint64_t n,
+                     int seed, pa_extent *out);
+
+/* k-mer prefilter of src/DistPWRunner.cpp:48-54: keep[i] = 0 when
+ * count <= ckthr || kscore <= mosthr, else 1.  Returns the kept count. */
+int pa_kmer_prune(const pa_pair *pairs, uint64_t n, int ckthr, float mosthr,
+                  uint8_t *keep, uint64_t *kept);
+
+/* ---- device memory helpers (for the _dev entry points) --------------- */
+int pa_device_alloc(struct pa_ctx *ctx, size_t bytes, void **ptr);
+int pa_device_free(struct pa_ctx *ctx, void *ptr);
+int pa_memcpy_h2d(struct pa_ctx *ctx, void *dst, const void *src, size_t bytes);
+int pa_memcpy_d2h(struct pa_ctx *ctx, void *dst, const void *src, size_t bytes);
+int pa_synchronize(struct pa_ctx *ctx);
+
+/* ---- counters -------------------------------------------------------- */
+int pa_get_stats(struct pa_ctx *ctx, pa_stats *out);
+int pa_reset_stats(struct pa_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PASTIS_HIP_H */

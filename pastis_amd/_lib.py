@@ -1,0 +1,120 @@
+"""ctypes binding of libpastis_hip.so (the C ABI of include/pastis_hip.h).
+
+The library is built in-tree by __graft_entry__.build().  There is no fallback:
+a missing or unloadable library raises immediately.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpastis_hip.so")
+GEN_PATH = os.path.join(HERE, "libpastis_gen.so")
+
+PA_OK = 0
+PA_MODE_SW, PA_MODE_XDROP = 0, 1
+PA_SIDE_ROW, PA_SIDE_COL, PA_SIDE_BOTH = 0, 1, 2
+PA_ENC_CODES, PA_ENC_ASCII = 0, 1
+PA_GAP_AS_CALLED, PA_GAP_AS_DOCUMENTED = 0, 1
+PA_FLAG_PASS, PA_FLAG_RERUN32, PA_FLAG_EMPTY = 1, 2, 4
+
+# Every extern "C" function of include/pastis_hip.h.
+EXPORTS = [
+    "pa_create", "pa_config_for_cli", "pa_destroy", "pa_strerror",
+    "pa_last_error", "pa_load_seqs", "pa_align", "pa_align_dev",
+    "pa_sw_scores", "pa_sw_scores_dev", "pa_xdrop_extents", "pa_kmer_prune",
+    "pa_device_alloc", "pa_device_free", "pa_memcpy_h2d", "pa_memcpy_d2h",
+    "pa_synchronize", "pa_get_stats", "pa_reset_stats",
+]
+
+PAIR_DTYPE = np.dtype([("r", "<u4"), ("c", "<u4"), ("count", "<u2"),
+                       ("kscore", "<i2"), ("seed", "<u2", (4,))])
+RESULT_DTYPE = np.dtype([("score", "<i4"), ("r_beg", "<u2"), ("r_end", "<u2"),
+                         ("c_beg", "<u2"), ("c_end", "<u2"),
+                         ("matches", "<u4"), ("cols", "<u4"),
+                         ("gap_opens", "<u4"), ("identity", "<f4"),
+                         ("score_aln", "<f4"), ("flags", "<u2"),
+                         ("seed", "<u2")])
+SCORE_DTYPE = np.dtype([("score", "<i4"), ("r_end", "<u2"), ("c_end", "<u2")])
+EXTENT_DTYPE = np.dtype([("r_beg", "<u2"), ("r_end", "<u2"), ("c_beg", "<u2"),
+                         ("c_end", "<u2")])
+
+
+class PaConfig(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("gap_open", C.c_int32),
+                ("gap_ext", C.c_int32), ("xdrop", C.c_int32),
+                ("xdrop_gap", C.c_int32), ("seed_len", C.c_int32),
+                ("seed_cnt", C.c_int32), ("ani_thr", C.c_int32),
+                ("cov_thr", C.c_double), ("device", C.c_int32),
+                ("flags", C.c_int32), ("matrix", C.c_void_p)]
+
+
+class PaStats(C.Structure):
+    _fields_ = [("pairs", C.c_uint64), ("dp_cells", C.c_uint64),
+                ("xdrop_cells", C.c_uint64), ("rerun_pairs", C.c_uint64),
+                ("ms_sw", C.c_double), ("ms_xdrop", C.c_double),
+                ("ms_nw", C.c_double), ("ms_trace", C.c_double),
+                ("ms_total", C.c_double), ("launches", C.c_uint64)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+assert PAIR_DTYPE.itemsize == 20 and RESULT_DTYPE.itemsize == 36
+assert SCORE_DTYPE.itemsize == 8 and EXTENT_DTYPE.itemsize == 8
+
+_lib = None
+
+
+class PaError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libpastis_hip.so; raises if it is missing (no CPU fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libpastis_hip.so is not built: run "
+                          "__graft_entry__.build() (expected at %s)" % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    vp, u64, i32 = C.c_void_p, C.c_uint64, C.c_int
+    L.pa_create.argtypes = [C.POINTER(PaConfig), C.POINTER(vp)]
+    L.pa_config_for_cli.argtypes = [C.POINTER(PaConfig), i32, i32, i32, i32,
+                                    i32, i32]
+    L.pa_destroy.argtypes = [vp]
+    L.pa_destroy.restype = None
+    L.pa_strerror.argtypes = [i32]
+    L.pa_strerror.restype = C.c_char_p
+    L.pa_last_error.argtypes = [vp]
+    L.pa_last_error.restype = C.c_char_p
+    L.pa_load_seqs.argtypes = [vp, i32, vp, vp, u64, i32]
+    for f in ("pa_align", "pa_sw_scores"):
+        getattr(L, f).argtypes = [vp, vp, u64, vp]
+    for f in ("pa_align_dev", "pa_sw_scores_dev"):
+        getattr(L, f).argtypes = [vp, vp, u64, vp, vp]
+    L.pa_xdrop_extents.argtypes = [vp, vp, u64, i32, vp]
+    L.pa_kmer_prune.argtypes = [vp, u64, i32, C.c_float, vp,
+                                C.POINTER(C.c_uint64)]
+    L.pa_device_alloc.argtypes = [vp, C.c_size_t, C.POINTER(vp)]
+    L.pa_device_free.argtypes = [vp, vp]
+    L.pa_memcpy_h2d.argtypes = [vp, vp, vp, C.c_size_t]
+    L.pa_memcpy_d2h.argtypes = [vp, vp, vp, C.c_size_t]
+    L.pa_synchronize.argtypes = [vp]
+    L.pa_get_stats.argtypes = [vp, C.POINTER(PaStats)]
+    L.pa_reset_stats.argtypes = [vp]
+    _lib = L
+    return L
+
+
+def check(ctx, rc, what):
+    if rc != PA_OK:
+        L = lib()
+        msg = L.pa_strerror(rc).decode()
+        if ctx:
+            detail = L.pa_last_error(ctx).decode()
+            if detail:
+                msg += ": " + detail
+        raise PaError("%s failed (%d): %s" % (what, rc, msg))

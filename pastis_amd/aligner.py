@@ -1,0 +1,138 @@
+"""Python face of the aligner: a thin owner of one pa_ctx.
+
+The reference's operator is the C++ class pastis::PWAlign
+(inc/align/PWAlign.hpp:20-82); its MI355X replacement is HipAligner
+(pastis_amd/csrc/HipAligner.hpp) over the C ABI of include/pastis_hip.h.  This
+module drives the same C ABI from Python for tests and benchmarks.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import (EXTENT_DTYPE, PAIR_DTYPE, RESULT_DTYPE, SCORE_DTYPE,
+                   PaConfig, PaStats, check, lib)
+
+ALPHABET = "ARNDCQEGHILKMFPSTWYVBZX*J"
+
+
+def config_for_cli(gap_open=-11, gap_ext=-2, klength=6, xdrop=0,
+                   seed_count=2, gap_model="as_called"):
+    """pa_config from the reference's CLI values (src/main.cpp:158-176)."""
+    cfg = PaConfig()
+    model = {"as_called": _lib.PA_GAP_AS_CALLED,
+             "as_documented": _lib.PA_GAP_AS_DOCUMENTED}[gap_model]
+    check(None, lib().pa_config_for_cli(C.byref(cfg), gap_open, gap_ext,
+                                        klength, xdrop, seed_count, model),
+          "pa_config_for_cli")
+    return cfg
+
+
+def _ptr(a):
+    return C.c_void_p(a.ctypes.data) if a is not None and a.size else None
+
+
+class Aligner:
+    """Owns a pa_ctx: sequences are loaded once, batches aligned many times."""
+
+    def __init__(self, cfg=None, **kw):
+        self.cfg = cfg if cfg is not None else config_for_cli(**kw)
+        self._ctx = C.c_void_p()
+        check(None, lib().pa_create(C.byref(self.cfg), C.byref(self._ctx)),
+              "pa_create")
+
+    @property
+    def ctx(self):
+        return self._ctx
+
+    def close(self):
+        if self._ctx:
+            lib().pa_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def load_seqs(self, codes, offsets, side=_lib.PA_SIDE_BOTH,
+                  encoding=_lib.PA_ENC_CODES):
+        codes = np.ascontiguousarray(codes, np.uint8)
+        offsets = np.ascontiguousarray(offsets, np.uint64)
+        check(self._ctx, lib().pa_load_seqs(self._ctx, side, _ptr(codes),
+                                            _ptr(offsets), len(offsets) - 1,
+                                            encoding), "pa_load_seqs")
+
+    def align(self, pairs):
+        pairs = np.ascontiguousarray(pairs, PAIR_DTYPE)
+        out = np.zeros(len(pairs), RESULT_DTYPE)
+        check(self._ctx, lib().pa_align(self._ctx, _ptr(pairs), len(pairs),
+                                        _ptr(out)), "pa_align")
+        return out
+
+    def sw_scores(self, pairs):
+        pairs = np.ascontiguousarray(pairs, PAIR_DTYPE)
+        out = np.zeros(len(pairs), SCORE_DTYPE)
+        check(self._ctx, lib().pa_sw_scores(self._ctx, _ptr(pairs), len(pairs),
+                                            _ptr(out)), "pa_sw_scores")
+        return out
+
+    def xdrop_extents(self, pairs, seed=0):
+        pairs = np.ascontiguousarray(pairs, PAIR_DTYPE)
+        out = np.zeros(len(pairs), EXTENT_DTYPE)
+        check(self._ctx, lib().pa_xdrop_extents(self._ctx, _ptr(pairs),
+                                                len(pairs), seed, _ptr(out)),
+              "pa_xdrop_extents")
+        return out
+
+    # device-resident batches (benchmarks)
+    def device_alloc(self, nbytes):
+        p = C.c_void_p()
+        check(self._ctx, lib().pa_device_alloc(self._ctx, nbytes, C.byref(p)),
+              "pa_device_alloc")
+        return p
+
+    def device_free(self, p):
+        check(self._ctx, lib().pa_device_free(self._ctx, p), "pa_device_free")
+
+    def h2d(self, dst, arr):
+        arr = np.ascontiguousarray(arr)
+        check(self._ctx, lib().pa_memcpy_h2d(self._ctx, dst, _ptr(arr),
+                                             arr.nbytes), "pa_memcpy_h2d")
+
+    def d2h(self, arr, src):
+        check(self._ctx, lib().pa_memcpy_d2h(self._ctx, _ptr(arr), src,
+                                             arr.nbytes), "pa_memcpy_d2h")
+        return arr
+
+    def align_dev(self, d_pairs, n, d_out):
+        check(self._ctx, lib().pa_align_dev(self._ctx, d_pairs, n, d_out, None),
+              "pa_align_dev")
+
+    def sw_scores_dev(self, d_pairs, n, d_out):
+        check(self._ctx, lib().pa_sw_scores_dev(self._ctx, d_pairs, n, d_out,
+                                                None), "pa_sw_scores_dev")
+
+    def synchronize(self):
+        check(self._ctx, lib().pa_synchronize(self._ctx), "pa_synchronize")
+
+    def stats(self):
+        s = PaStats()
+        check(self._ctx, lib().pa_get_stats(self._ctx, C.byref(s)),
+              "pa_get_stats")
+        return s.as_dict()
+
+    def reset_stats(self):
+        check(self._ctx, lib().pa_reset_stats(self._ctx), "pa_reset_stats")
+
+
+def kmer_prune(pairs, ckthr, mosthr):
+    """DistPWRunner.cpp:48-54 k-mer prefilter: boolean keep mask."""
+    pairs = np.ascontiguousarray(pairs, PAIR_DTYPE)
+    keep = np.zeros(len(pairs), np.uint8)
+    kept = C.c_uint64()
+    check(None, lib().pa_kmer_prune(_ptr(pairs), len(pairs), ckthr,
+                                    C.c_float(mosthr), _ptr(keep),
+                                    C.byref(kept)), "pa_kmer_prune")
+    return keep.astype(bool)

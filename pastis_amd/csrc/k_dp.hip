@@ -1,0 +1,693 @@
+// k_dp.hip -- wavefront dynamic-programming kernels for gfx950.
+//
+// One DP task per wavefront.  The row sequence (SeqAn's vertical sequence,
+// seqsr) is split over the 64 lanes in strips of 2R rows: the upper R rows
+// ("lo") and the lower R rows ("hi") of a lane share one register per row,
+// packed int16 lo/hi, and the hi half runs one column behind the lo half so
+// both halves advance with the same instruction.  Lane l works on column
+// t-2l (lo) and t-2l-1 (hi) at step t; the H and F values of its last row go
+// to lane l+1 through a DPP wave_shr:1 move.  Columns (the horizontal
+// sequence, seqsc) stream through; a pass covers 128R rows and longer row
+// sequences take several passes linked by a boundary row in global memory.
+//
+// The substitution scores come from a query profile in LDS: for each residue
+// code c and each row, score(row residue, c) + 128 as a byte.
+//
+// Recurrences (Gotoh; rows i over the row sequence, columns j over the
+// column sequence; gap of length L costs open + (L-1)*ext):
+//   E(i,j) = max(E(i,j-1) + ext, H(i,j-1) + open)        ties keep extension
+//   F(i,j) = max(F(i-1,j) + ext, H(i-1,j) + open)        ties keep extension
+//   H(i,j) = max(H(i-1,j-1) + s, F(i,j), E(i,j) [, 0 local])
+//            ties: restart (local) > diagonal > F (vertical) > E (horizontal)
+// Local mode reports the first maximum in column-major order (smaller column,
+// then smaller row), as SeqAn's column-wise fill with a strict '>' update.
+//
+// Traced kernels work on 4*value + tag so that one max picks the value and
+// the predecessor at once (tags: 3 restart, 2 diagonal, 1 F, 0 E); they store
+// a 4-bit code per cell (source, E extended, F extended) for the traceback.
+#include "pa_internal.h"
+
+namespace pa {
+namespace {
+
+typedef short s2v __attribute__((ext_vector_type(2)));
+
+template <bool I16> struct Ops;
+
+// Two int16 lanes in one VGPR: x = lo half-strip row, y = hi half-strip row.
+template <> struct Ops<true> {
+    typedef s2v V;
+    typedef uint32_t Key;
+    __device__ static inline V mk(int lo, int hi)
+    {
+        V v;
+        v.x = (short)lo;
+        v.y = (short)hi;
+        return v;
+    }
+    __device__ static inline V splat(int x) { return mk(x, x); }
+    __device__ static inline V add(V a, V b)
+    {
+        return __builtin_elementwise_add_sat(a, b);
+    }
+    __device__ static inline V max(V a, V b)
+    {
+        return __builtin_elementwise_max(a, b);
+    }
+    __device__ static inline uint32_t u(V a)
+    {
+        return __builtin_bit_cast(uint32_t, a);
+    }
+    __device__ static inline V v(uint32_t x) { return __builtin_bit_cast(V, x); }
+    __device__ static inline V andm(V a, int m)
+    {
+        uint32_t mm = ((uint32_t)m & 0xffffu) * 0x10001u;
+        return v(u(a) & mm);
+    }
+    __device__ static inline V orm(V a, int m)
+    {
+        uint32_t mm = ((uint32_t)m & 0xffffu) * 0x10001u;
+        return v(u(a) | mm);
+    }
+    __device__ static inline int lo(V a) { return a.x; }
+    __device__ static inline int hi(V a) { return a.y; }
+    template <int K>
+    __device__ static inline V prof(uint32_t lw, uint32_t hw)
+    {
+        // byte K of lw -> low half, byte K of hw -> high half, zero-extended
+        const uint32_t sel = (uint32_t)K | (0x0cu << 8) |
+                             ((uint32_t)(4 + K) << 16) | (0x0cu << 24);
+        uint32_t x = __builtin_amdgcn_perm(hw, lw, sel);
+        return v(x) + splat(-128);
+    }
+    // new row-above input: lo <- hi of lane-1, hi <- own lo
+    __device__ static inline V shuffle_up(V out)
+    {
+        uint32_t o = u(out);
+        uint32_t sh = (uint32_t)__builtin_amdgcn_update_dpp(
+            0, (int)o, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
+        return v(__builtin_amdgcn_alignbit(o, sh, 16));
+    }
+    __device__ static inline V set_lo(V a, int x)
+    {
+        a.x = (short)x;
+        return a;
+    }
+    // bits 0-3 of each half: the code nibble of that half
+    __device__ static inline uint32_t code(V hp, V ep, V fp)
+    {
+        return (u(hp) & 0x00030003u) | ((u(ep) & 0x00010001u) << 2) |
+               ((u(fp) & 0x00010001u) << 3);
+    }
+    // replace the halves selected by (ilo, ihi) of old with those of nw
+    __device__ static inline V blend(V old, V nw, bool ilo, bool ihi)
+    {
+        uint32_t m = (ilo ? 0x0000ffffu : 0u) | (ihi ? 0xffff0000u : 0u);
+        return v((u(nw) & m) | (u(old) & ~m));
+    }
+    __device__ static inline Key key(int score, int col)
+    {
+        return ((uint32_t)score << 16) | (uint32_t)(0xffff - col);
+    }
+    __device__ static inline int key_score(Key k) { return (int)(k >> 16); }
+    __device__ static inline int key_col(Key k)
+    {
+        return 0xffff - (int)(k & 0xffffu);
+    }
+    static constexpr int kNeg = -16384;
+    static constexpr int kNegTraced = -32000;
+};
+
+struct I2 {
+    int x, y;
+};
+
+// Two int32 values per row (the exact re-run and the long global tasks).
+template <> struct Ops<false> {
+    typedef I2 V;
+    typedef uint64_t Key;
+    __device__ static inline V mk(int lo, int hi) { return I2{lo, hi}; }
+    __device__ static inline V splat(int x) { return I2{x, x}; }
+    __device__ static inline V add(V a, V b) { return I2{a.x + b.x, a.y + b.y}; }
+    __device__ static inline V max(V a, V b)
+    {
+        return I2{a.x > b.x ? a.x : b.x, a.y > b.y ? a.y : b.y};
+    }
+    __device__ static inline V andm(V a, int m) { return I2{a.x & m, a.y & m}; }
+    __device__ static inline V orm(V a, int m) { return I2{a.x | m, a.y | m}; }
+    __device__ static inline int lo(V a) { return a.x; }
+    __device__ static inline int hi(V a) { return a.y; }
+    template <int K>
+    __device__ static inline V prof(uint32_t lw, uint32_t hw)
+    {
+        return I2{(int)((lw >> (8 * K)) & 0xffu) - 128,
+                  (int)((hw >> (8 * K)) & 0xffu) - 128};
+    }
+    __device__ static inline V shuffle_up(V out)
+    {
+        int sh = __builtin_amdgcn_update_dpp(0, out.y, 0x138, 0xf, 0xf, false);
+        return I2{sh, out.x};
+    }
+    __device__ static inline V set_lo(V a, int x)
+    {
+        a.x = x;
+        return a;
+    }
+    __device__ static inline uint32_t code(V hp, V ep, V fp)
+    {
+        uint32_t lo = ((uint32_t)hp.x & 3u) | (((uint32_t)ep.x & 1u) << 2) |
+                      (((uint32_t)fp.x & 1u) << 3);
+        uint32_t hi = ((uint32_t)hp.y & 3u) | (((uint32_t)ep.y & 1u) << 2) |
+                      (((uint32_t)fp.y & 1u) << 3);
+        return lo | (hi << 16);
+    }
+    __device__ static inline V blend(V old, V nw, bool ilo, bool ihi)
+    {
+        return I2{ilo ? nw.x : old.x, ihi ? nw.y : old.y};
+    }
+    __device__ static inline Key key(int score, int col)
+    {
+        return ((uint64_t)(uint32_t)score << 16) | (uint64_t)(0xffff - col);
+    }
+    __device__ static inline int key_score(Key k) { return (int)(k >> 16); }
+    __device__ static inline int key_col(Key k)
+    {
+        return 0xffff - (int)(k & 0xffffu);
+    }
+    static constexpr int kNeg = -(1 << 28);
+    static constexpr int kNegTraced = -(1 << 28);
+};
+
+template <int R> struct TraceWord;
+template <> struct TraceWord<2> {
+    typedef uint16_t T;
+    __device__ static inline T pack(const uint32_t *c)
+    {
+        uint32_t a = c[0] | (c[1] << 4);
+        return (T)((a & 0xffu) | ((a >> 8) & 0xff00u));
+    }
+};
+template <> struct TraceWord<4> {
+    typedef uint32_t T;
+    __device__ static inline T pack(const uint32_t *c)
+    {
+        return c[0] | (c[1] << 4) | (c[2] << 8) | (c[3] << 12);
+    }
+};
+template <> struct TraceWord<8> {
+    typedef uint2 T;
+    __device__ static inline T pack(const uint32_t *c)
+    {
+        uint32_t a = c[0] | (c[1] << 4) | (c[2] << 8) | (c[3] << 12);
+        uint32_t b = c[4] | (c[5] << 4) | (c[6] << 8) | (c[7] << 12);
+        return make_uint2((a & 0xffffu) | (b << 16),
+                          (a >> 16) | (b & 0xffff0000u));
+    }
+};
+
+template <int R> struct ProfWords;
+template <> struct ProfWords<2> {
+    __device__ static inline void load(const uint8_t *p, uint32_t *w)
+    {
+        w[0] = *(const uint16_t *)p;
+    }
+};
+template <> struct ProfWords<4> {
+    __device__ static inline void load(const uint8_t *p, uint32_t *w)
+    {
+        w[0] = *(const uint32_t *)p;
+    }
+};
+template <> struct ProfWords<8> {
+    __device__ static inline void load(const uint8_t *p, uint32_t *w)
+    {
+        uint2 x = *(const uint2 *)p;
+        w[0] = x.x;
+        w[1] = x.y;
+    }
+};
+
+template <class O, int R, int K>
+struct ProfUnpack {
+    __device__ static inline void run(const uint32_t *lw, const uint32_t *hw,
+                                      typename O::V *s)
+    {
+        ProfUnpack<O, R, K - 1>::run(lw, hw, s);
+        s[K - 1] = O::template prof<(K - 1) % 4>(lw[(K - 1) / 4],
+                                                 hw[(K - 1) / 4]);
+    }
+};
+template <class O, int R>
+struct ProfUnpack<O, R, 0> {
+    __device__ static inline void run(const uint32_t *, const uint32_t *,
+                                      typename O::V *)
+    {
+    }
+};
+
+__device__ inline uint64_t wave_max_u64(uint64_t v)
+{
+    for (int off = 32; off >= 1; off >>= 1) {
+        uint32_t lo = __shfl_xor((uint32_t)v, off);
+        uint32_t hi = __shfl_xor((uint32_t)(v >> 32), off);
+        uint64_t o = ((uint64_t)hi << 32) | lo;
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+template <int R, bool LOCAL, bool TRACE, bool I16>
+__global__ __launch_bounds__(64) void k_dp(DpArgs a)
+{
+    typedef Ops<I16> O;
+    typedef typename O::V V;
+    typedef typename O::Key Key;
+    constexpr int ROWS = 128 * R;
+    constexpr int SH = TRACE ? 2 : 0;
+    constexpr int NEG = TRACE ? O::kNegTraced : O::kNeg;
+    extern __shared__ uint32_t prof32[];
+    const uint8_t *prof = (const uint8_t *)prof32;
+    const int lane = threadIdx.x;
+    const uint32_t count = *a.count;
+    const int open = a.gap_open * (1 << SH);
+    const int ext = a.gap_ext * (1 << SH);
+    const int gmax = open > ext ? open : ext;
+    const V OPEN = O::splat(open);
+    const V EXT = O::splat(TRACE ? ext + 1 : ext);
+    int32_t *bnd = a.bnd + (size_t)blockIdx.x * a.bnd_stride;
+
+    for (;;) {
+        uint32_t k = 0;
+        if (lane == 0)
+            k = atomicAdd(a.queue, 1u);
+        k = __builtin_amdgcn_readfirstlane(k);
+        if (k >= count)
+            break;
+        const uint32_t tid = a.list[k];
+        const DpTask T = a.tasks[tid];
+        const uint8_t *qs = (const uint8_t *)(uintptr_t)T.q_off;
+        const uint8_t *ts = (const uint8_t *)(uintptr_t)T.t_off;
+        const int qlen = (int)T.qlen, tlen = (int)T.tlen;
+        const int npass = (qlen + ROWS - 1) / ROWS;
+        uint8_t *tr = nullptr;
+        if (TRACE)
+            tr = a.trace + a.trace_off[tid];
+
+        Key bk_lo = O::key(0, 0), bk_hi = O::key(0, 0);
+        int bp_lo = 0, bp_hi = 0;
+        V saved[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            saved[r] = O::splat(0);
+        int gscore = 0;
+
+        for (int p = 0; p < npass; ++p) {
+            const int i0 = p * ROWS;
+            __syncthreads();
+            // query profile of rows [i0, i0 + ROWS)
+            for (int idx = lane; idx < kProfChars * ROWS / 4; idx += 64) {
+                const int c = idx / (ROWS / 4);
+                const int r4 = (idx % (ROWS / 4)) * 4;
+                uint32_t w = 0;
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const int row = i0 + r4 + b;
+                    int val = -128;
+                    if (c < kAlpha && row < qlen) {
+                        const int s = a.mat[(int)qs[row] * 32 + c];
+                        val = TRACE ? 4 * s + 2 : s;
+                    }
+                    w |= (uint32_t)(val + 128) << (8 * b);
+                }
+                prof32[idx] = w;
+            }
+            if (p > 0)
+                __threadfence();
+            __syncthreads();
+
+            const int rlo = i0 + 2 * R * lane;
+            const bool active = rlo < qlen;
+            int nact = (qlen - i0 + 2 * R - 1) / (2 * R);
+            nact = nact > 64 ? 64 : nact;
+            V H[R], E[R], Ho[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int ilo = rlo + r + 1, ihi = rlo + R + r + 1;
+                const int hlo = LOCAL ? 0 : open + (ilo - 1) * gmax;
+                const int hhi = LOCAL ? 0 : open + (ihi - 1) * gmax;
+                H[r] = O::mk(hlo, hhi);
+                E[r] = O::splat(NEG);
+                Ho[r] = O::add(H[r], OPEN);
+            }
+            V outH = H[R - 1];
+            V outF = LOCAL ? O::splat(NEG) : H[R - 1];
+            // inputs of step 1: row above at column t (up) and t-1 (diag)
+            int bh0, bh1, bf1;
+            if (p == 0) {
+                bh0 = 0;
+                bh1 = LOCAL ? 0 : open;
+                bf1 = NEG;
+            } else {
+                bh0 = bnd[0];
+                bh1 = bnd[2];
+                bf1 = bnd[3];
+            }
+            V upH = O::shuffle_up(outH), upF = O::shuffle_up(outF);
+            V dg = O::splat(NEG);
+            if (lane == 0) {
+                upH = O::set_lo(upH, bh1);
+                upF = O::set_lo(upF, bf1);
+                dg = O::set_lo(dg, bh0);
+            }
+            int c_lo = kSentinel, c_hi = kSentinel;
+            const int last_step = tlen + 2 * (nact - 1) + 1;
+            // cell (qlen, tlen) for the global score
+            const int qrow = qlen - 1 - i0;
+            const int cap_lane = qrow / (2 * R);
+            const int cap_half = (qrow % (2 * R)) / R;
+            const int cap_r = qrow % R;
+            const int cap_step = tlen + 2 * cap_lane + cap_half;
+            const bool last_pass = p == npass - 1;
+
+            for (int t = 1; t <= last_step; ++t) {
+                // column residues: lane 0 reads, the others take lane-1's
+                // residue of the previous step (two columns back for them)
+                {
+                    int nc = __builtin_amdgcn_update_dpp(0, c_hi, 0x138, 0xf,
+                                                         0xf, false);
+                    c_hi = c_lo;
+                    if (lane == 0)
+                        nc = t <= tlen ? (int)ts[t - 1] : kSentinel;
+                    c_lo = nc;
+                }
+                int nbh = 0, nbf = NEG;  // boundary for lane 0 at step t+1
+                if (lane == 0) {
+                    const int jn = t + 1;
+                    if (p == 0) {
+                        nbh = LOCAL ? 0 : open + (jn - 1) * gmax;
+                    } else if (jn <= tlen) {
+                        nbh = bnd[2 * jn];
+                        nbf = bnd[2 * jn + 1];
+                    }
+                }
+                const int jlo = t - 2 * lane;
+                const int jhi = jlo - 1;
+                if (active && jlo >= 1 && jlo <= tlen + 1) {
+                    uint32_t lw[(R + 3) / 4], hw[(R + 3) / 4];
+                    ProfWords<R>::load(prof + c_lo * ROWS + 2 * R * lane, lw);
+                    ProfWords<R>::load(prof + c_hi * ROWS + 2 * R * lane + R,
+                                       hw);
+                    V s[R];
+                    ProfUnpack<O, R, R>::run(lw, hw, s);
+                    V hdiag = dg, hup = upH, fup = upF;
+                    V hoUp = O::add(upH, OPEN);
+                    V cm = O::splat(0);
+                    uint32_t codes[R];
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        if (!TRACE) {
+                            V e = O::max(O::add(E[r], EXT), Ho[r]);
+                            V f = O::max(O::add(fup, EXT), hoUp);
+                            V hd = O::add(hdiag, s[r]);
+                            V h = O::max(O::max(hd, e), f);
+                            if (LOCAL)
+                                h = O::max(h, O::splat(0));
+                            hdiag = H[r];
+                            H[r] = h;
+                            E[r] = e;
+                            Ho[r] = O::add(h, OPEN);
+                            hoUp = Ho[r];
+                            hup = h;
+                            fup = f;
+                            if (LOCAL)
+                                cm = O::max(cm, h);
+                        } else {
+                            V ep = O::max(O::add(E[r], EXT), Ho[r]);
+                            V e = O::andm(ep, ~3);
+                            V fp = O::max(O::add(fup, EXT), hoUp);
+                            V f = O::andm(fp, ~3);
+                            V hd = O::add(hdiag, s[r]);
+                            V hp = O::max(O::max(hd, O::orm(f, 1)), e);
+                            if (LOCAL)
+                                hp = O::max(hp, O::splat(3));
+                            V h = O::andm(hp, ~3);
+                            codes[r] = O::code(hp, ep, fp);
+                            hdiag = H[r];
+                            H[r] = h;
+                            E[r] = e;
+                            Ho[r] = O::add(h, OPEN);
+                            hoUp = Ho[r];
+                            hup = h;
+                            fup = f;
+                            if (LOCAL)
+                                cm = O::max(cm, h);
+                        }
+                    }
+                    outH = hup;
+                    outF = fup;
+                    if (TRACE) {
+                        typedef typename TraceWord<R>::T W;
+                        W w = TraceWord<R>::pack(codes);
+                        const size_t rec =
+                            ((size_t)p * (size_t)(tlen + 130) + (size_t)(t - 1)) *
+                                64u + (size_t)lane;
+                        *(W *)(tr + rec * R) = w;
+                    }
+                    if (LOCAL) {
+                        const bool vlo = jlo <= tlen;
+                        const bool vhi = jhi >= 1 && jhi <= tlen;
+                        const Key klo = vlo ? O::key(O::lo(cm) >> SH, jlo) : (Key)0;
+                        const Key khi = vhi ? O::key(O::hi(cm) >> SH, jhi) : (Key)0;
+                        const bool ilo = klo > bk_lo, ihi = khi > bk_hi;
+                        bk_lo = ilo ? klo : bk_lo;
+                        bk_hi = ihi ? khi : bk_hi;
+                        bp_lo = ilo ? p : bp_lo;
+                        bp_hi = ihi ? p : bp_hi;
+                        if (ilo || ihi) {
+#pragma unroll
+                            for (int r = 0; r < R; ++r)
+                                saved[r] = O::blend(saved[r], H[r], ilo, ihi);
+                        }
+                    } else if (last_pass && lane == cap_lane && t == cap_step) {
+#pragma unroll
+                        for (int r = 0; r < R; ++r)
+                            if (r == cap_r)
+                                gscore = cap_half ? O::hi(H[r]) : O::lo(H[r]);
+                    }
+                    if (p + 1 < npass && lane == 63 && jhi >= 0 && jhi <= tlen) {
+                        bnd[2 * jhi] = O::hi(outH);
+                        bnd[2 * jhi + 1] = O::hi(outF);
+                    }
+                }
+                dg = upH;
+                upH = O::shuffle_up(outH);
+                upF = O::shuffle_up(outF);
+                if (lane == 0) {
+                    upH = O::set_lo(upH, nbh);
+                    upF = O::set_lo(upF, nbf);
+                }
+            }
+        }
+
+        // ---- results ----------------------------------------------------
+        DpOut o = {};
+        if (LOCAL) {
+            // first row of each half whose saved value equals its best score
+            uint64_t best = 0;
+            for (int hf = 0; hf < 2; ++hf) {
+                const Key bk = hf ? bk_hi : bk_lo;
+                const int bp = hf ? bp_hi : bp_lo;
+                const int sc = O::key_score(bk);
+                if (sc <= 0)
+                    continue;
+                const int col = O::key_col(bk);
+                const int rbase = bp * ROWS + 2 * R * lane + hf * R;
+                int row = -1;
+#pragma unroll
+                for (int r = R - 1; r >= 0; --r) {
+                    const int vv = hf ? O::hi(saved[r]) : O::lo(saved[r]);
+                    if ((vv >> SH) == sc && rbase + r < qlen)
+                        row = rbase + r;
+                }
+                if (row < 0)
+                    continue;
+                const uint64_t ck = ((uint64_t)(uint32_t)sc << 40) |
+                                    ((uint64_t)(0xffff - col) << 24) |
+                                    (uint64_t)(0xffffff - row);
+                best = ck > best ? ck : best;
+            }
+            best = wave_max_u64(best);
+            const int sc = (int)(best >> 40);
+            if (sc > 0) {
+                o.score = sc;
+                o.t_end = 0xffff - (int)((best >> 24) & 0xffff);
+                o.q_end = 0xffffff - (int)(best & 0xffffff) + 1;
+            }
+            if (I16 && lane == 0) {
+                const int lim = TRACE ? 8190 : 32766;
+                if (sc >= lim) {
+                    uint32_t pos = atomicAdd(a.sat_count, 1u);
+                    a.sat_list[pos] = tid;
+                }
+            }
+        } else {
+            // the capturing lane of the last pass holds H(qlen, tlen)
+            const int qrow = qlen - 1 - (npass - 1) * ROWS;
+            const int cl = qrow / (2 * R);
+            int g = __shfl(gscore, cl);
+            o.score = g >> SH;
+            o.q_end = qlen;
+            o.t_end = tlen;
+        }
+        if (lane == 0) {
+            o.q_beg = o.q_end;
+            o.t_beg = o.t_end;
+            a.out[tid] = o;
+        }
+    }
+}
+
+}  // namespace
+}  // namespace pa
+
+// ---------------------------------------------------------------------------
+// Traceback over the stored 4-bit codes, one task per thread.  Walks back
+// from the end cell (local) or (qlen, tlen) (global) under the same priority
+// the DP used, and counts what computeAlignmentStats counts: columns,
+// identical residue pairs and maximal gap runs in either row.
+// ---------------------------------------------------------------------------
+namespace pa {
+namespace {
+
+__device__ inline int trace_code(const uint8_t *tr, int R, int tlen, int i,
+                                 int j)
+{
+    const int rows = 128 * R;
+    const int i0 = i - 1;
+    const int p = i0 / rows, rr = i0 % rows;
+    const int l = rr / (2 * R), q = rr % (2 * R);
+    const int half = q / R, r = q % R;
+    const int t = j + 2 * l + half;
+    const size_t rec =
+        ((size_t)p * (size_t)(tlen + 130) + (size_t)(t - 1)) * 64u + (size_t)l;
+    const int nib = half * R + r;
+    const uint8_t b = tr[rec * R + (nib >> 1)];
+    return (nib & 1) ? (b >> 4) : (b & 15);
+}
+
+__global__ __launch_bounds__(256) void k_traceback(TbArgs a)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= *a.count)
+        return;
+    const uint32_t tid = a.list[k];
+    const DpTask T = a.tasks[tid];
+    DpOut o = a.out[tid];
+    const uint8_t *qs = (const uint8_t *)(uintptr_t)T.q_off;
+    const uint8_t *ts = (const uint8_t *)(uintptr_t)T.t_off;
+    const int tlen = (int)T.tlen;
+    const int R = rows_per_lane(T.qlen);
+    const uint8_t *tr = a.trace + a.trace_off[tid];
+    int i = o.q_end, j = o.t_end;
+    int matches = 0, cols = 0, runs = 0;
+    int last = 0;   // 1 diagonal, 2 vertical, 3 horizontal
+    int state = 0;  // 0 H, 1 E (horizontal), 2 F (vertical)
+    if (a.local && o.score <= 0) {
+        i = j = 0;
+    } else {
+        for (;;) {
+            if (i == 0 || j == 0) {
+                if (a.local)
+                    break;
+                // global boundary: a single gap run to the origin
+                if (i > 0) {
+                    cols += i;
+                    runs += last != 2;
+                    i = 0;
+                } else if (j > 0) {
+                    cols += j;
+                    runs += last != 3;
+                    j = 0;
+                }
+                break;
+            }
+            const int c = trace_code(tr, R, tlen, i, j);
+            if (state == 0) {
+                const int src = c & 3;
+                if (src == 3)
+                    break;  // restart (local)
+                if (src == 2) {
+                    ++cols;
+                    matches += qs[i - 1] == ts[j - 1];
+                    last = 1;
+                    --i;
+                    --j;
+                } else {
+                    state = src == 1 ? 2 : 1;
+                }
+            } else if (state == 1) {
+                ++cols;
+                runs += last != 3;
+                last = 3;
+                state = (c >> 2) & 1 ? 1 : 0;
+                --j;
+            } else {
+                ++cols;
+                runs += last != 2;
+                last = 2;
+                state = (c >> 3) & 1 ? 2 : 0;
+                --i;
+            }
+        }
+    }
+    o.q_beg = i;
+    o.t_beg = j;
+    o.matches = matches;
+    o.cols = cols;
+    o.gap_opens = runs;
+    a.out[tid] = o;
+}
+
+template <int R, bool L, bool T, bool I>
+hipError_t launch_one(const DpArgs &a, int grid, hipStream_t s)
+{
+    const size_t lds = (size_t)kProfChars * 128 * R;
+    hipLaunchKernelGGL((k_dp<R, L, T, I>), dim3(grid), dim3(64), lds, s, a);
+    return hipGetLastError();
+}
+
+template <int R>
+hipError_t launch_r(bool local, bool trace, bool i16, const DpArgs &a,
+                    int grid, hipStream_t s)
+{
+    if (local) {
+        if (trace)
+            return i16 ? launch_one<R, true, true, true>(a, grid, s)
+                       : launch_one<R, true, true, false>(a, grid, s);
+        return i16 ? launch_one<R, true, false, true>(a, grid, s)
+                   : launch_one<R, true, false, false>(a, grid, s);
+    }
+    return i16 ? launch_one<R, false, true, true>(a, grid, s)
+               : launch_one<R, false, true, false>(a, grid, s);
+}
+
+}  // namespace
+
+hipError_t launch_dp(int R, bool local, bool trace, bool i16, const DpArgs &a,
+                     int grid, hipStream_t s)
+{
+    switch (R) {
+    case 2: return launch_r<2>(local, trace, i16, a, grid, s);
+    case 4: return launch_r<4>(local, trace, i16, a, grid, s);
+    case 8: return launch_r<8>(local, trace, i16, a, grid, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_traceback(const TbArgs &a, int grid, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_traceback, dim3(grid), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace pa

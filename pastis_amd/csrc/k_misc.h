@@ -1,0 +1,44 @@
+// k_misc.h -- launchers of the task, binning and result kernels.
+#pragma once
+
+#include "pa_internal.h"
+
+namespace pa {
+
+// Bins: 2 * R-class (R = 2, 4, 8) + (1 for the int32 kernels).
+constexpr int kBins = 6;
+
+hipError_t fill_u32(uint32_t *p, uint32_t v, uint32_t n, hipStream_t s);
+hipError_t build_sw_tasks(const pa_pair *pairs, uint64_t n,
+                          const uint64_t *roffs, const uint64_t *coffs,
+                          DpTask *tasks, unsigned long long *cells,
+                          hipStream_t s);
+hipError_t build_xd_tasks(const pa_pair *pairs, uint64_t n, int seed_cnt,
+                          const uint64_t *roffs, const uint64_t *coffs,
+                          XdTask *tasks, hipStream_t s);
+hipError_t build_nw_tasks(const XdTask *xt, uint64_t nt, const XdOut *xo,
+                          DpTask *tasks, unsigned long long *cells,
+                          unsigned long long *xcells, hipStream_t s);
+hipError_t bin_tasks(const DpTask *tasks, uint32_t a, uint32_t b, int local,
+                     int maxs, int open, int ext, uint32_t *lists,
+                     uint32_t cap, uint32_t *counts, hipStream_t s);
+hipError_t trace_sizes(const DpTask *tasks, uint64_t n, uint64_t *sizes,
+                       hipStream_t s);
+hipError_t empty_out(const DpTask *tasks, uint32_t a, uint32_t b, int local,
+                     int open, int ext, DpOut *out, hipStream_t s);
+hipError_t mark_flags(const uint32_t *list, const uint32_t *count,
+                      uint32_t max_n, uint8_t *flags, uint8_t bit,
+                      hipStream_t s);
+hipError_t finalize_sw(const DpTask *tasks, const DpOut *dp,
+                       const uint8_t *tflags, uint64_t n, double cov_thr,
+                       int ani_thr, pa_result *out, hipStream_t s);
+hipError_t finalize_xd(const XdTask *xt, const XdOut *xo, const DpOut *dp,
+                       const uint8_t *tflags, uint64_t n, int seed_cnt,
+                       double cov_thr, int ani_thr, pa_result *out,
+                       hipStream_t s);
+hipError_t scores_out(const DpOut *dp, uint64_t n, pa_score *out,
+                      hipStream_t s);
+hipError_t extents_out(const XdOut *xo, uint64_t n, int seed, pa_extent *out,
+                       hipStream_t s);
+
+}  // namespace pa

@@ -1,0 +1,460 @@
+// k_misc.hip -- task construction, binning and result kernels.
+#include "k_misc.h"
+
+namespace pa {
+namespace {
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = v;
+}
+
+// Pair -> local-alignment task: rows = row sequence (SeqAn V), columns =
+// column sequence (SeqAn H), as localAlignment(seqsc, seqsr) at
+// SeqanFullAligner.cpp:231.
+__global__ void k_build_sw_tasks(const pa_pair *__restrict__ pairs, uint64_t n,
+                                 const uint64_t *__restrict__ roffs,
+                                 const uint64_t *__restrict__ coffs,
+                                 DpTask *__restrict__ tasks,
+                                 unsigned long long *cells)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t c = 0;
+    if (i < n) {
+        const pa_pair p = pairs[i];
+        DpTask t;
+        t.q_off = roffs[p.r];
+        t.qlen = (uint32_t)(roffs[p.r + 1] - roffs[p.r]);
+        t.t_off = coffs[p.c];
+        t.tlen = (uint32_t)(coffs[p.c + 1] - coffs[p.c]);
+        t.id = (uint32_t)i;
+        t.flags = 0;
+        tasks[i] = t;
+        c = (uint64_t)t.qlen * t.tlen;
+    }
+    // one atomic per wavefront
+    for (int off = 32; off >= 1; off >>= 1)
+        c += __shfl_xor(c, off);
+    if ((threadIdx.x & 63) == 0 && c)
+        atomicAdd(cells, (unsigned long long)c);
+}
+
+// Pair x seed -> extension task (TSeed(col offset, row offset, k),
+// SeqanXdropAligner.cpp:273-280).
+__global__ void k_build_xd_tasks(const pa_pair *__restrict__ pairs, uint64_t n,
+                                 int seed_cnt,
+                                 const uint64_t *__restrict__ roffs,
+                                 const uint64_t *__restrict__ coffs,
+                                 XdTask *__restrict__ tasks)
+{
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n * (uint64_t)seed_cnt)
+        return;
+    const uint64_t i = k / seed_cnt;
+    const int s = (int)(k % seed_cnt);
+    const pa_pair p = pairs[i];
+    XdTask t;
+    t.v_off = roffs[p.r];
+    t.nv = (uint32_t)(roffs[p.r + 1] - roffs[p.r]);
+    t.h_off = coffs[p.c];
+    t.nh = (uint32_t)(coffs[p.c + 1] - coffs[p.c]);
+    t.beg_v = s == 0 ? p.seed[0] : p.seed[2];
+    t.beg_h = s == 0 ? p.seed[1] : p.seed[3];
+    t.id = (uint32_t)(i * 2 + s);
+    t.pad = 0;
+    tasks[k] = t;
+}
+
+// Extension -> global alignment of the extended infixes
+// (globalAlignment(seqsc_ex, seqsr_ex), SeqanXdropAligner.cpp:284-296).
+__global__ void k_build_nw_tasks(const XdTask *__restrict__ xt, uint64_t nt,
+                                 const XdOut *__restrict__ xo,
+                                 DpTask *__restrict__ tasks,
+                                 unsigned long long *cells,
+                                 unsigned long long *xcells)
+{
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t c = 0, xc = 0;
+    if (k < nt) {
+        const XdTask x = xt[k];
+        const XdOut o = xo[x.id];
+        DpTask t;
+        t.q_off = x.v_off + (uint64_t)o.beg_v;
+        t.qlen = (uint32_t)(o.end_v - o.beg_v);
+        t.t_off = x.h_off + (uint64_t)o.beg_h;
+        t.tlen = (uint32_t)(o.end_h - o.beg_h);
+        t.id = x.id;
+        t.flags = 0;
+        tasks[k] = t;
+        c = (uint64_t)t.qlen * t.tlen;
+        xc = o.cells;
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+        c += __shfl_xor(c, off);
+        xc += __shfl_xor(xc, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (c)
+            atomicAdd(cells, (unsigned long long)c);
+        if (xc)
+            atomicAdd(xcells, (unsigned long long)xc);
+    }
+}
+
+__device__ inline bool global_i16_ok(uint32_t q, uint32_t t, int maxs,
+                                     int open, int ext)
+{
+    // traced int16 kernels hold 4*value + tag in [-32000, 32767]
+    const int gmax = open > ext ? open : ext;
+    const int mn = q < t ? (int)q : (int)t;
+    const long ub = (long)maxs * mn;
+    const long lb = 2L * open + (long)(q + t) * gmax + (ext < open ? ext : open);
+    return 4 * ub + 3 < 32760 && 4 * lb > -31000;
+}
+
+// Tasks [a, b) -> per-bin lists.  Bin = 2*R-class + (int32 ? 1 : 0).
+__global__ void k_bin(const DpTask *__restrict__ tasks, uint32_t a, uint32_t b,
+                      int local, int maxs, int open, int ext,
+                      uint32_t *__restrict__ lists, uint32_t cap,
+                      uint32_t *counts)
+{
+    const uint32_t i = a + blockIdx.x * blockDim.x + threadIdx.x;
+    int bin = -1;
+    if (i < b) {
+        const DpTask t = tasks[i];
+        if (t.qlen > 0 && t.tlen > 0) {
+            const int R = rows_per_lane(t.qlen);
+            const int rc = R == 2 ? 0 : (R == 4 ? 1 : 2);
+            const bool i16 =
+                local ? true : global_i16_ok(t.qlen, t.tlen, maxs, open, ext);
+            bin = 2 * rc + (i16 ? 0 : 1);
+        }
+    }
+    const int lane = threadIdx.x & 63;
+    for (int bb = 0; bb < kBins; ++bb) {
+        const uint64_t m = __ballot(bin == bb);
+        if (!m)
+            continue;
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        uint32_t base = 0;
+        if (lane == leader)
+            base = atomicAdd(&counts[bb], (uint32_t)__popcll(m));
+        base = __shfl(base, leader);
+        if (bin == bb) {
+            const uint32_t rank =
+                __popcll(m & ((1ull << lane) - 1ull));
+            lists[(size_t)bb * cap + base + rank] = i;
+        }
+    }
+}
+
+__global__ void k_trace_sizes(const DpTask *__restrict__ tasks, uint64_t n,
+                              uint64_t *__restrict__ sizes)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const DpTask t = tasks[i];
+        sizes[i] = (t.qlen && t.tlen)
+                       ? trace_bytes(t.qlen, t.tlen, rows_per_lane(t.qlen))
+                       : 0;
+    }
+}
+
+// Empty tasks (a zero-length span) never reach a DP kernel.
+__global__ void k_empty_out(const DpTask *__restrict__ tasks, uint32_t a,
+                            uint32_t b, int local, int open, int ext,
+                            DpOut *out)
+{
+    const uint32_t i = a + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b)
+        return;
+    const DpTask t = tasks[i];
+    if (t.qlen && t.tlen)
+        return;
+    DpOut o = {};
+    if (!local) {
+        const int L = (int)(t.qlen + t.tlen);
+        const int gmax = open > ext ? open : ext;
+        o.score = L ? open + (L - 1) * gmax : 0;
+        o.q_end = (int)t.qlen;
+        o.t_end = (int)t.tlen;
+        o.cols = L;
+        o.gap_opens = L ? 1 : 0;
+    }
+    out[i] = o;
+}
+
+__global__ void k_mark(const uint32_t *list, const uint32_t *count,
+                       uint8_t *flags, uint8_t bit)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < *count)
+        flags[list[k]] |= bit;
+}
+
+__device__ inline float identity_of(uint32_t matches, uint32_t cols)
+{
+    return (float)(100.0 * (double)(float)matches / (double)(float)cols);
+}
+
+// SeqanFullAligner.cpp:247-261 / SeqanXdropAligner.cpp:338-351
+__device__ inline bool predicate(uint32_t cols, uint32_t gap_opens,
+                                 uint32_t matches, int len_c, int len_r,
+                                 double cov_thr, int ani_thr, float *ident)
+{
+    if (cols == 0) {
+        *ident = __int_as_float(0x7fc00000);
+        return false;
+    }
+    const float id = identity_of(matches, cols);
+    *ident = id;
+    const double amg = (double)((int)cols - (int)gap_opens);
+    const double c1 = amg / len_c, c2 = amg / len_r;
+    const double cov = c1 > c2 ? c1 : c2;
+    return cov >= cov_thr && id >= (float)ani_thr;
+}
+
+__global__ void k_finalize_sw(const DpTask *__restrict__ tasks,
+                              const DpOut *__restrict__ dp,
+                              const uint8_t *__restrict__ tflags, uint64_t n,
+                              double cov_thr, int ani_thr,
+                              pa_result *__restrict__ out)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const DpTask t = tasks[i];
+    const DpOut o = dp[i];
+    pa_result r = {};
+    r.score = o.score;
+    r.r_beg = (uint16_t)o.q_beg;
+    r.r_end = (uint16_t)o.q_end;
+    r.c_beg = (uint16_t)o.t_beg;
+    r.c_end = (uint16_t)o.t_end;
+    r.matches = (uint32_t)o.matches;
+    r.cols = (uint32_t)o.cols;
+    r.gap_opens = (uint32_t)o.gap_opens;
+    float id;
+    const bool pass = predicate(r.cols, r.gap_opens, r.matches, (int)t.tlen,
+                                (int)t.qlen, cov_thr, ani_thr, &id);
+    r.identity = id;
+    r.score_aln = pass ? id / 100.0f : 0.0f;
+    r.flags = (uint16_t)((pass ? PA_FLAG_PASS : 0) |
+                         ((tflags[i] & 1) ? PA_FLAG_RERUN32 : 0) |
+                         (o.score == 0 ? PA_FLAG_EMPTY : 0));
+    r.seed = 0;
+    out[i] = r;
+}
+
+// Best of the seeds by identity, strict '>' (SeqanXdropAligner.cpp:303-323),
+// predicate on the full sequence lengths (:338-351).
+__global__ void k_finalize_xd(const XdTask *__restrict__ xt,
+                              const XdOut *__restrict__ xo,
+                              const DpOut *__restrict__ dp,
+                              const uint8_t *__restrict__ tflags, uint64_t n,
+                              int seed_cnt, double cov_thr, int ani_thr,
+                              pa_result *__restrict__ out)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    int best = 0;
+    float best_id = 0.0f;
+    for (int s = 0; s < seed_cnt; ++s) {
+        const DpOut o = dp[i * seed_cnt + s];
+        const float id = o.cols ? identity_of((uint32_t)o.matches,
+                                              (uint32_t)o.cols)
+                                : __int_as_float(0x7fc00000);
+        if (s == 0 || id > best_id) {
+            best = s;
+            best_id = id;
+        }
+    }
+    const uint64_t k = i * seed_cnt + best;
+    const DpOut o = dp[k];
+    const XdTask x = xt[i * seed_cnt + best];
+    const XdOut e = xo[i * 2 + best];
+    pa_result r = {};
+    r.score = o.score;
+    r.r_beg = (uint16_t)e.beg_v;
+    r.r_end = (uint16_t)e.end_v;
+    r.c_beg = (uint16_t)e.beg_h;
+    r.c_end = (uint16_t)e.end_h;
+    r.matches = (uint32_t)o.matches;
+    r.cols = (uint32_t)o.cols;
+    r.gap_opens = (uint32_t)o.gap_opens;
+    float id;
+    const bool pass = predicate(r.cols, r.gap_opens, r.matches, (int)x.nh,
+                                (int)x.nv, cov_thr, ani_thr, &id);
+    r.identity = id;
+    r.score_aln = pass ? id / 100.0f : 0.0f;
+    r.flags = (uint16_t)((pass ? PA_FLAG_PASS : 0) |
+                         ((tflags[k] & 1) ? PA_FLAG_RERUN32 : 0));
+    r.seed = (uint16_t)best;
+    out[i] = r;
+}
+
+__global__ void k_scores_out(const DpOut *__restrict__ dp, uint64_t n,
+                             pa_score *__restrict__ out)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const DpOut o = dp[i];
+    pa_score s;
+    s.score = o.score;
+    s.r_end = (uint16_t)o.q_end;
+    s.c_end = (uint16_t)o.t_end;
+    out[i] = s;
+}
+
+__global__ void k_extents_out(const XdOut *__restrict__ xo, uint64_t n,
+                              int seed, pa_extent *__restrict__ out)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const XdOut e = xo[i * 2 + seed];
+    pa_extent x;
+    x.r_beg = (uint16_t)e.beg_v;
+    x.r_end = (uint16_t)e.end_v;
+    x.c_beg = (uint16_t)e.beg_h;
+    x.c_end = (uint16_t)e.end_h;
+    out[i] = x;
+}
+
+inline unsigned blocks(uint64_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+hipError_t fill_u32(uint32_t *p, uint32_t v, uint32_t n, hipStream_t s)
+{
+    if (!n)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks(n, 256)), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t build_sw_tasks(const pa_pair *pairs, uint64_t n,
+                          const uint64_t *roffs, const uint64_t *coffs,
+                          DpTask *tasks, unsigned long long *cells,
+                          hipStream_t s)
+{
+    if (!n)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_build_sw_tasks, dim3(blocks(n, 256)), dim3(256), 0, s,
+                       pairs, n, roffs, coffs, tasks, cells);
+    return hipGetLastError();
+}
+
+hipError_t build_xd_tasks(const pa_pair *pairs, uint64_t n, int seed_cnt,
+                          const uint64_t *roffs, const uint64_t *coffs,
+                          XdTask *tasks, hipStream_t s)
+{
+    if (!n)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_build_xd_tasks, dim3(blocks(n * seed_cnt, 256)),
+                       dim3(256), 0, s, pairs, n, seed_cnt, roffs, coffs, tasks);
+    return hipGetLastError();
+}
+
+hipError_t build_nw_tasks(const XdTask *xt, uint64_t nt, const XdOut *xo,
+                          DpTask *tasks, unsigned long long *cells,
+                          unsigned long long *xcells, hipStream_t s)
+{
+    if (!nt)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_build_nw_tasks, dim3(blocks(nt, 256)), dim3(256), 0, s,
+                       xt, nt, xo, tasks, cells, xcells);
+    return hipGetLastError();
+}
+
+hipError_t bin_tasks(const DpTask *tasks, uint32_t a, uint32_t b, int local,
+                     int maxs, int open, int ext, uint32_t *lists,
+                     uint32_t cap, uint32_t *counts, hipStream_t s)
+{
+    if (b <= a)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_bin, dim3(blocks(b - a, 256)), dim3(256), 0, s, tasks,
+                       a, b, local, maxs, open, ext, lists, cap, counts);
+    return hipGetLastError();
+}
+
+hipError_t trace_sizes(const DpTask *tasks, uint64_t n, uint64_t *sizes,
+                       hipStream_t s)
+{
+    if (!n)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_trace_sizes, dim3(blocks(n, 256)), dim3(256), 0, s,
+                       tasks, n, sizes);
+    return hipGetLastError();
+}
+
+hipError_t empty_out(const DpTask *tasks, uint32_t a, uint32_t b, int local,
+                     int open, int ext, DpOut *out, hipStream_t s)
+{
+    if (b <= a)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_empty_out, dim3(blocks(b - a, 256)), dim3(256), 0, s,
+                       tasks, a, b, local, open, ext, out);
+    return hipGetLastError();
+}
+
+hipError_t mark_flags(const uint32_t *list, const uint32_t *count,
+                      uint32_t max_n, uint8_t *flags, uint8_t bit,
+                      hipStream_t s)
+{
+    if (!max_n)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_mark, dim3(blocks(max_n, 256)), dim3(256), 0, s, list,
+                       count, flags, bit);
+    return hipGetLastError();
+}
+
+hipError_t finalize_sw(const DpTask *tasks, const DpOut *dp,
+                       const uint8_t *tflags, uint64_t n, double cov_thr,
+                       int ani_thr, pa_result *out, hipStream_t s)
+{
+    if (!n)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_finalize_sw, dim3(blocks(n, 256)), dim3(256), 0, s,
+                       tasks, dp, tflags, n, cov_thr, ani_thr, out);
+    return hipGetLastError();
+}
+
+hipError_t finalize_xd(const XdTask *xt, const XdOut *xo, const DpOut *dp,
+                       const uint8_t *tflags,
+                       uint64_t n, int seed_cnt, double cov_thr, int ani_thr,
+                       pa_result *out, hipStream_t s)
+{
+    if (!n)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_finalize_xd, dim3(blocks(n, 256)), dim3(256), 0, s,
+                       xt, xo, dp, tflags, n, seed_cnt, cov_thr,
+                       ani_thr, out);
+    return hipGetLastError();
+}
+
+hipError_t scores_out(const DpOut *dp, uint64_t n, pa_score *out,
+                      hipStream_t s)
+{
+    if (!n)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_scores_out, dim3(blocks(n, 256)), dim3(256), 0, s, dp,
+                       n, out);
+    return hipGetLastError();
+}
+
+hipError_t extents_out(const XdOut *xo, uint64_t n, int seed, pa_extent *out,
+                       hipStream_t s)
+{
+    if (!n)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_extents_out, dim3(blocks(n, 256)), dim3(256), 0, s, xo,
+                       n, seed, out);
+    return hipGetLastError();
+}
+
+}  // namespace pa

@@ -1,0 +1,1006 @@
+// pa_api.hip -- host side of the C ABI declared in include/pastis_hip.h.
+//
+// A context owns one HIP stream, the uploaded row/column sequence sets and a
+// workspace that grows on demand and is reused across batches.  Every batch
+// runs as a fixed sequence of kernels on the context's stream:
+//   SW mode   : build tasks -> bin -> traced local DP (int16, int32 re-run of
+//               saturated tasks) -> traceback -> predicate
+//   X-drop    : build seed tasks -> X-drop extension -> build infix tasks ->
+//               bin -> traced global DP -> traceback -> best seed + predicate
+// The traced DP writes 4-bit codes into a trace store; batches whose trace
+// would exceed the store are split into chunks at task boundaries.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "k_misc.h"
+
+using namespace pa;
+
+namespace {
+
+struct Buf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+struct Timer {
+    int kind;  // 0 sw, 1 xdrop, 2 nw, 3 trace, 4 total
+    hipEvent_t a, b;
+};
+
+const char *kStatus[] = {"ok",
+                         "invalid argument",
+                         "HIP runtime error",
+                         "out of memory",
+                         "sequence longer than 65535",
+                         "residue code outside 0..24",
+                         "pair index outside the loaded sequences",
+                         "seed outside its sequences",
+                         "sequences not loaded",
+                         "no usable gfx950 device",
+                         "size limit exceeded"};
+
+const char kAlphabet[] = "ARNDCQEGHILKMFPSTWYVBZX*J";
+
+// inc/ScoreMat.hpp:104-131 (BLOSUM62, letter order of src/Alphabet.cpp:26)
+const int8_t kBlosum62[25 * 25] = {
+     4,-1,-2,-2, 0,-1,-1, 0,-2,-1,-1,-1,-1,-2,-1, 1, 0,-3,-2, 0,-2,-1, 0,-4,-1,
+    -1, 5, 0,-2,-3, 1, 0,-2, 0,-3,-2, 2,-1,-3,-2,-1,-1,-3,-2,-3,-1, 0,-1,-4,-2,
+    -2, 0, 6, 1,-3, 0, 0, 0, 1,-3,-3, 0,-2,-3,-2, 1, 0,-4,-2,-3, 3, 0,-1,-4,-3,
+    -2,-2, 1, 6,-3, 0, 2,-1,-1,-3,-4,-1,-3,-3,-1, 0,-1,-4,-3,-3, 4, 1,-1,-4,-4,
+     0,-3,-3,-3, 9,-3,-4,-3,-3,-1,-1,-3,-1,-2,-3,-1,-1,-2,-2,-1,-3,-3,-2,-4,-1,
+    -1, 1, 0, 0,-3, 5, 2,-2, 0,-3,-2, 1, 0,-3,-1, 0,-1,-2,-1,-2, 0, 3,-1,-4,-2,
+    -1, 0, 0, 2,-4, 2, 5,-2, 0,-3,-3, 1,-2,-3,-1, 0,-1,-3,-2,-2, 1, 4,-1,-4,-3,
+     0,-2, 0,-1,-3,-2,-2, 6,-2,-4,-4,-2,-3,-3,-2, 0,-2,-2,-3,-3,-1,-2,-1,-4,-4,
+    -2, 0, 1,-1,-3, 0, 0,-2, 8,-3,-3,-1,-2,-1,-2,-1,-2,-2, 2,-3, 0, 0,-1,-4,-3,
+    -1,-3,-3,-3,-1,-3,-3,-4,-3, 4, 2,-3, 1, 0,-3,-2,-1,-3,-1, 3,-3,-3,-1,-4, 2,
+    -1,-2,-3,-4,-1,-2,-3,-4,-3, 2, 4,-2, 2, 0,-3,-2,-1,-2,-1, 1,-4,-3,-1,-4, 4,
+    -1, 2, 0,-1,-3, 1, 1,-2,-1,-3,-2, 5,-1,-3,-1, 0,-1,-3,-2,-2, 0, 1,-1,-4,-2,
+    -1,-1,-2,-3,-1, 0,-2,-3,-2, 1, 2,-1, 5, 0,-2,-1,-1,-1,-1, 1,-3,-1,-1,-4, 2,
+    -2,-3,-3,-3,-2,-3,-3,-3,-1, 0, 0,-3, 0, 6,-4,-2,-2, 1, 3,-1,-3,-3,-1,-4, 0,
+    -1,-2,-2,-1,-3,-1,-1,-2,-2,-3,-3,-1,-2,-4, 7,-1,-1,-4,-3,-2,-2,-1,-2,-4,-3,
+     1,-1, 1, 0,-1, 0, 0, 0,-1,-2,-2, 0,-1,-2,-1, 4, 1,-3,-2,-2, 0, 0, 0,-4,-2,
+     0,-1, 0,-1,-1,-1,-1,-2,-2,-1,-1,-1,-1,-2,-1, 1, 5,-2,-2, 0,-1,-1, 0,-4,-1,
+    -3,-3,-4,-4,-2,-2,-3,-2,-2,-3,-2,-3,-1, 1,-4,-3,-2,11, 2,-3,-4,-3,-2,-4,-2,
+    -2,-2,-2,-3,-2,-1,-2,-3, 2,-1,-1,-2,-1, 3,-3,-2,-2, 2, 7,-1,-3,-2,-1,-4,-1,
+     0,-3,-3,-3,-1,-2,-2,-3,-3, 3, 1,-2, 1,-1,-2,-2, 0,-3,-1, 4,-3,-2,-1,-4, 1,
+    -2,-1, 3, 4,-3, 0, 1,-1, 0,-3,-4, 0,-3,-3,-2, 0,-1,-4,-3,-3, 4, 1,-1,-4,-4,
+    -1, 0, 0, 1,-3, 3, 4,-2, 0,-3,-3, 1,-1,-3,-1, 0,-1,-3,-2,-2, 1, 4,-1,-4,-3,
+     0,-1,-1,-1,-2,-1,-1,-1,-1,-1,-1,-1,-1,-1,-2, 0, 0,-2,-1,-1,-1,-1,-1,-4,-1,
+    -4,-4,-4,-4,-4,-4,-4,-4,-4,-4,-4,-4,-4,-4,-4,-4,-4,-4,-4,-4,-4,-4,-4, 1,-4,
+    -1,-2,-3,-4,-1,-2,-3,-4,-3, 2, 4,-2, 2, 0,-3,-2,-1,-2,-1, 1,-4,-3,-1,-4, 4,
+};
+
+// Work-queue and counter slots (uint32) of the per-chunk counter block.
+enum {
+    C_BIN = 0,          // kBins list lengths
+    C_BINQ = 6,         // kBins work-queue heads
+    C_SAT = 12,         // 3 saturation-list lengths (per R class)
+    C_SATQ = 15,        // 3 re-run queue heads
+    C_XDQ = 18,         // X-drop queue head
+    C_XDOVF = 19,       // X-drop overflow count
+    C_XDOVFQ = 20,      // X-drop overflow queue head
+    C_XDN = 21,         // X-drop task count
+    C_XDDUMMY = 22,     // overflow sink of the wide re-run
+    C_ERR = 23,         // validation errors
+    C_N = 32
+};
+
+constexpr int kDpGrid = 2048;      // persistent DP workgroups (one wave each)
+constexpr int kXdBlocks = 512;     // X-drop workgroups of 256 lanes
+constexpr uint32_t kXdCap = 256;   // anti-diagonal slice per lane
+constexpr int kXdWideLanes = 4096; // lanes of the wide re-run
+
+}  // namespace
+
+struct pa_ctx {
+    pa_config cfg{};
+    int8_t mat[32 * 32];
+    int maxs = 0;
+    int device = 0;
+    hipStream_t st = nullptr;
+    std::string err;
+    // sequence sets
+    Buf rcodes, ccodes, roffs, coffs;
+    bool shared_cols = false;
+    std::vector<uint32_t> rlen, clen;
+    uint32_t max_rlen = 0, max_clen = 0;
+    bool have_rows = false, have_cols = false;
+    // workspace
+    Buf pairs, results, scores, tasks, dpout, tflags, sizes, toffs, lists,
+        sat, ctr, trace, bnd, xdt, xdo, xds, ovf, cub, cells, dmat, ext;
+    size_t trace_cap = 0;
+    std::vector<uint64_t> h_toffs;
+    std::vector<Timer> pending;
+    std::vector<hipEvent_t> free_events;
+    pa_stats stats{};
+};
+
+namespace {
+
+int fail(pa_ctx *c, int code, const std::string &msg)
+{
+    if (c)
+        c->err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, x)                                                       \
+    do {                                                                     \
+        hipError_t e_ = (x);                                                 \
+        if (e_ != hipSuccess)                                                \
+            return fail((ctx), PA_ERR_HIP,                                   \
+                        std::string(#x) + ": " + hipGetErrorString(e_));     \
+    } while (0)
+
+int grow(pa_ctx *c, Buf &b, size_t bytes)
+{
+    if (b.cap >= bytes && b.p)
+        return PA_OK;
+    if (b.p)
+        hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+    size_t want = bytes + bytes / 4 + 256;
+    if (hipMalloc(&b.p, want) != hipSuccess) {
+        b.p = nullptr;
+        return fail(c, PA_ERR_NOMEM,
+                    "device allocation of " + std::to_string(want) + " bytes");
+    }
+    b.cap = want;
+    return PA_OK;
+}
+
+void release(Buf &b)
+{
+    if (b.p)
+        hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+}
+
+template <class T> T *P(Buf &b) { return (T *)b.p; }
+
+hipEvent_t get_event(pa_ctx *c)
+{
+    if (!c->free_events.empty()) {
+        hipEvent_t e = c->free_events.back();
+        c->free_events.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    hipEventCreate(&e);
+    return e;
+}
+
+struct Scope {
+    pa_ctx *c;
+    Timer t;
+    hipStream_t s;
+    Scope(pa_ctx *c_, int kind, hipStream_t s_) : c(c_), s(s_)
+    {
+        t.kind = kind;
+        t.a = get_event(c);
+        t.b = get_event(c);
+        hipEventRecord(t.a, s);
+    }
+    ~Scope()
+    {
+        hipEventRecord(t.b, s);
+        c->pending.push_back(t);
+    }
+};
+
+void resolve_timers(pa_ctx *c)
+{
+    for (auto &t : c->pending) {
+        float ms = 0.f;
+        hipEventSynchronize(t.b);
+        hipEventElapsedTime(&ms, t.a, t.b);
+        switch (t.kind) {
+        case 0: c->stats.ms_sw += ms; break;
+        case 1: c->stats.ms_xdrop += ms; break;
+        case 2: c->stats.ms_nw += ms; break;
+        case 3: c->stats.ms_trace += ms; break;
+        default: c->stats.ms_total += ms; break;
+        }
+        c->free_events.push_back(t.a);
+        c->free_events.push_back(t.b);
+    }
+    c->pending.clear();
+}
+
+int ensure_common(pa_ctx *c, uint64_t nt)
+{
+    int rc;
+    if ((rc = grow(c, c->ctr, C_N * sizeof(uint32_t))))
+        return rc;
+    if ((rc = grow(c, c->cells, 4 * sizeof(uint64_t))))
+        return rc;
+    if ((rc = grow(c, c->tasks, nt * sizeof(DpTask))))
+        return rc;
+    if ((rc = grow(c, c->dpout, nt * sizeof(DpOut))))
+        return rc;
+    if ((rc = grow(c, c->tflags, nt)))
+        return rc;
+    if ((rc = grow(c, c->lists, (size_t)kBins * nt * sizeof(uint32_t))))
+        return rc;
+    if ((rc = grow(c, c->sat, 3 * nt * sizeof(uint32_t))))
+        return rc;
+    return PA_OK;
+}
+
+int ensure_bnd(pa_ctx *c)
+{
+    const size_t need =
+        (size_t)kDpGrid * 2u * (size_t)(c->max_clen + 2) * sizeof(int32_t);
+    return grow(c, c->bnd, need);
+}
+
+// Runs the DP kernels (and the traceback when traced) on tasks [0, nt).
+int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
+           hipStream_t s)
+{
+    int rc;
+    if ((rc = ensure_bnd(c)))
+        return rc;
+    HIPCHK(c, hipMemsetAsync(c->tflags.p, 0, nt, s));
+    std::vector<std::pair<uint64_t, uint64_t>> chunks;
+    if (trace) {
+        if ((rc = grow(c, c->sizes, nt * sizeof(uint64_t) + 8)))
+            return rc;
+        if ((rc = grow(c, c->toffs, nt * sizeof(uint64_t) + 8)))
+            return rc;
+        HIPCHK(c, trace_sizes(P<DpTask>(c->tasks), nt, P<uint64_t>(c->sizes), s));
+        size_t tmp = 0;
+        HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(
+                      nullptr, tmp, P<uint64_t>(c->sizes),
+                      P<uint64_t>(c->toffs), (int)(nt + 1), s));
+        if ((rc = grow(c, c->cub, tmp)))
+            return rc;
+        // sizes[nt] = 0 so that toffs[nt] is the total
+        HIPCHK(c, hipMemsetAsync(P<uint64_t>(c->sizes) + nt, 0, 8, s));
+        HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(
+                      c->cub.p, tmp, P<uint64_t>(c->sizes),
+                      P<uint64_t>(c->toffs), (int)(nt + 1), s));
+        uint64_t total = 0;
+        HIPCHK(c, hipMemcpyAsync(&total, P<uint64_t>(c->toffs) + nt, 8,
+                                 hipMemcpyDeviceToHost, s));
+        HIPCHK(c, hipStreamSynchronize(s));
+        if (total <= c->trace_cap) {
+            chunks.push_back({0, nt});
+            if ((rc = grow(c, c->trace, std::max<uint64_t>(total, 64))))
+                return rc;
+        } else {
+            if ((rc = grow(c, c->trace, c->trace_cap)))
+                return rc;
+            c->h_toffs.resize(nt + 1);
+            HIPCHK(c, hipMemcpyAsync(c->h_toffs.data(), c->toffs.p,
+                                     (nt + 1) * 8, hipMemcpyDeviceToHost, s));
+            HIPCHK(c, hipStreamSynchronize(s));
+            uint64_t a = 0;
+            while (a < nt) {
+                uint64_t b = a + 1;
+                if (c->h_toffs[b] - c->h_toffs[a] > c->trace_cap)
+                    return fail(c, PA_ERR_LIMIT,
+                                "one alignment needs more trace memory than "
+                                "the trace store holds");
+                // largest b with toffs[b] - toffs[a] <= cap
+                uint64_t lo = b, hi = nt;
+                while (lo < hi) {
+                    uint64_t mid = (lo + hi + 1) / 2;
+                    if (c->h_toffs[mid] - c->h_toffs[a] <= c->trace_cap)
+                        lo = mid;
+                    else
+                        hi = mid - 1;
+                }
+                chunks.push_back({a, lo});
+                a = lo;
+            }
+        }
+    } else {
+        chunks.push_back({0, nt});
+    }
+
+    uint32_t *ctr = P<uint32_t>(c->ctr);
+    uint32_t *lists = P<uint32_t>(c->lists);
+    uint32_t *sat = P<uint32_t>(c->sat);
+    for (auto &ch : chunks) {
+        const uint32_t a = (uint32_t)ch.first, b = (uint32_t)ch.second;
+        uint64_t base = 0;
+        if (trace && chunks.size() > 1)
+            base = c->h_toffs[a];
+        HIPCHK(c, fill_u32(ctr, 0, C_XDQ, s));
+        HIPCHK(c, bin_tasks(P<DpTask>(c->tasks), a, b, local ? 1 : 0, c->maxs,
+                            c->cfg.gap_open, c->cfg.gap_ext, lists,
+                            (uint32_t)nt, ctr + C_BIN, s));
+        HIPCHK(c, empty_out(P<DpTask>(c->tasks), a, b, local ? 1 : 0,
+                            c->cfg.gap_open, c->cfg.gap_ext,
+                            P<DpOut>(c->dpout), s));
+        DpArgs args{};
+        args.codes = nullptr;  // task offsets hold absolute device addresses
+        args.tasks = P<DpTask>(c->tasks);
+        args.mat = P<int8_t>(c->dmat);
+        args.out = P<DpOut>(c->dpout);
+        args.trace = trace ? P<uint8_t>(c->trace) - base : nullptr;
+        args.trace_off = trace ? P<uint64_t>(c->toffs) : nullptr;
+        args.bnd = P<int32_t>(c->bnd);
+        args.bnd_stride = 2u * (c->max_clen + 2);
+        args.gap_open = c->cfg.gap_open;
+        args.gap_ext = c->cfg.gap_ext;
+        const int grid = (int)std::min<uint64_t>(b - a, kDpGrid);
+        {
+            Scope sc(c, kind, s);
+            for (int rcl = 0; rcl < 3; ++rcl) {
+                const int R = rcl == 0 ? 2 : (rcl == 1 ? 4 : 8);
+                for (int w = 0; w < 2; ++w) {
+                    const int bin = 2 * rcl + w;
+                    args.list = lists + (size_t)bin * nt;
+                    args.count = ctr + C_BIN + bin;
+                    args.queue = ctr + C_BINQ + bin;
+                    args.sat_list = sat + (size_t)rcl * nt;
+                    args.sat_count = ctr + C_SAT + rcl;
+                    HIPCHK(c, launch_dp(R, local, trace, w == 0, args, grid, s));
+                    c->stats.launches++;
+                }
+                if (local) {
+                    // exact int32 re-run of the tasks whose int16 DP saturated
+                    args.list = sat + (size_t)rcl * nt;
+                    args.count = ctr + C_SAT + rcl;
+                    args.queue = ctr + C_SATQ + rcl;
+                    HIPCHK(c, launch_dp(R, local, trace, false, args, grid, s));
+                    HIPCHK(c, mark_flags(args.list, args.count, b - a,
+                                         P<uint8_t>(c->tflags), 1, s));
+                    c->stats.launches += 2;
+                }
+            }
+        }
+        if (trace) {
+            Scope sc(c, 3, s);
+            TbArgs tb{};
+            tb.codes = nullptr;
+            tb.tasks = P<DpTask>(c->tasks);
+            tb.trace = P<uint8_t>(c->trace) - base;
+            tb.trace_off = P<uint64_t>(c->toffs);
+            tb.out = P<DpOut>(c->dpout);
+            tb.local = local ? 1 : 0;
+            // all non-empty tasks of the chunk: the six bin lists
+            for (int bin = 0; bin < kBins; ++bin) {
+                tb.list = lists + (size_t)bin * nt;
+                tb.count = ctr + C_BIN + bin;
+                HIPCHK(c, launch_traceback(tb, (int)((b - a + 255) / 256), s));
+                c->stats.launches++;
+            }
+        }
+    }
+    return PA_OK;
+}
+
+int validate_pairs_host(pa_ctx *c, const pa_pair *pairs, uint64_t n)
+{
+    const int k = c->cfg.seed_len;
+    for (uint64_t i = 0; i < n; ++i) {
+        const pa_pair &p = pairs[i];
+        if (p.r >= c->rlen.size() || p.c >= c->clen.size())
+            return fail(c, PA_ERR_INDEX,
+                        "pair " + std::to_string(i) + " names sequence (" +
+                            std::to_string(p.r) + ", " + std::to_string(p.c) +
+                            ") outside the loaded sets");
+        if (c->cfg.mode == PA_MODE_XDROP) {
+            const uint32_t lr = c->rlen[p.r], lc = c->clen[p.c];
+            for (int s = 0; s < c->cfg.seed_cnt; ++s) {
+                const uint32_t ro = p.seed[2 * s], co = p.seed[2 * s + 1];
+                if (ro + (uint32_t)k > lr || co + (uint32_t)k > lc)
+                    return fail(c, PA_ERR_SEED,
+                                "pair " + std::to_string(i) + " seed " +
+                                    std::to_string(s) +
+                                    " does not fit in its sequences");
+            }
+        }
+    }
+    return PA_OK;
+}
+
+int check_ready(pa_ctx *c)
+{
+    if (!c)
+        return PA_ERR_ARG;
+    if (!c->have_rows || !c->have_cols)
+        return fail(c, PA_ERR_NOSEQ, "load row and column sequences first");
+    return PA_OK;
+}
+
+hipStream_t pick(pa_ctx *c, void *stream)
+{
+    return stream ? (hipStream_t)stream : c->st;
+}
+
+int align_sw_dev(pa_ctx *c, const pa_pair *d_pairs, uint64_t n,
+                 pa_result *d_out, hipStream_t s)
+{
+    int rc;
+    if ((rc = ensure_common(c, n)))
+        return rc;
+    Scope tot(c, 4, s);
+    HIPCHK(c, build_sw_tasks(d_pairs, n, P<uint64_t>(c->roffs),
+                             P<uint64_t>(c->coffs), P<DpTask>(c->tasks),
+                             (unsigned long long *)P<uint64_t>(c->cells), s));
+    if ((rc = run_dp(c, n, true, true, 0, s)))
+        return rc;
+    HIPCHK(c, finalize_sw(P<DpTask>(c->tasks), P<DpOut>(c->dpout),
+                          P<uint8_t>(c->tflags), n, c->cfg.cov_thr,
+                          c->cfg.ani_thr, d_out, s));
+    c->stats.pairs += n;
+    return PA_OK;
+}
+
+int run_xdrop(pa_ctx *c, const pa_pair *d_pairs, uint64_t n, hipStream_t s)
+{
+    int rc;
+    const int sc = c->cfg.seed_cnt;
+    const uint64_t nt = n * (uint64_t)sc;
+    if ((rc = grow(c, c->xdt, nt * sizeof(XdTask))))
+        return rc;
+    if ((rc = grow(c, c->xdo, 2 * n * sizeof(XdOut))))
+        return rc;
+    if ((rc = grow(c, c->ovf, nt * sizeof(uint32_t))))
+        return rc;
+    const size_t lanes = (size_t)kXdBlocks * 256;
+    if ((rc = grow(c, c->xds, lanes * 3 * kXdCap * sizeof(int32_t))))
+        return rc;
+    uint32_t *ctr = P<uint32_t>(c->ctr);
+    HIPCHK(c, fill_u32(ctr + C_XDQ, 0, C_N - C_XDQ, s));
+    HIPCHK(c, fill_u32(ctr + C_XDN, (uint32_t)nt, 1, s));
+    HIPCHK(c, build_xd_tasks(d_pairs, n, sc, P<uint64_t>(c->roffs),
+                             P<uint64_t>(c->coffs), P<XdTask>(c->xdt), s));
+    {
+        Scope t(c, 1, s);
+        const int blocks = (int)std::min<uint64_t>((nt + 255) / 256, kXdBlocks);
+        HIPCHK(c, launch_xdrop_k(P<XdTask>(c->xdt), nullptr, ctr + C_XDN,
+                                 ctr + C_XDQ, nullptr, P<int8_t>(c->dmat),
+                                 c->cfg.xdrop_gap, c->cfg.xdrop,
+                                 P<int32_t>(c->xds), kXdCap, P<XdOut>(c->xdo),
+                                 P<uint32_t>(c->ovf), ctr + C_XDOVF,
+                                 c->cfg.seed_len, blocks, s));
+        c->stats.launches++;
+    }
+    uint32_t novf = 0;
+    HIPCHK(c, hipMemcpyAsync(&novf, ctr + C_XDOVF, 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    if (novf) {
+        const uint32_t cap =
+            std::max(c->max_rlen, c->max_clen) + 4;
+        // every launched lane may take a task from the queue, so the scratch
+        // covers whole workgroups
+        const size_t blocks2 =
+            (std::min<size_t>(novf, kXdWideLanes) + 255) / 256;
+        const size_t lanes2 = blocks2 * 256;
+        Buf wide;
+        if ((rc = grow(c, wide, lanes2 * 3 * (size_t)cap * sizeof(int32_t))))
+            return rc;
+        {
+            Scope t(c, 1, s);
+            HIPCHK(c, launch_xdrop_k(P<XdTask>(c->xdt), P<uint32_t>(c->ovf),
+                                     ctr + C_XDOVF, ctr + C_XDOVFQ, nullptr,
+                                     P<int8_t>(c->dmat), c->cfg.xdrop_gap,
+                                     c->cfg.xdrop, P<int32_t>(wide), cap,
+                                     P<XdOut>(c->xdo), P<uint32_t>(c->ovf),
+                                     ctr + C_XDDUMMY, c->cfg.seed_len,
+                                     (int)blocks2, s));
+            c->stats.launches++;
+        }
+        HIPCHK(c, hipStreamSynchronize(s));
+        release(wide);
+    }
+    return PA_OK;
+}
+
+int align_xd_dev(pa_ctx *c, const pa_pair *d_pairs, uint64_t n,
+                 pa_result *d_out, hipStream_t s)
+{
+    int rc;
+    const uint64_t nt = n * (uint64_t)c->cfg.seed_cnt;
+    if ((rc = ensure_common(c, nt)))
+        return rc;
+    Scope tot(c, 4, s);
+    if ((rc = run_xdrop(c, d_pairs, n, s)))
+        return rc;
+    HIPCHK(c, build_nw_tasks(P<XdTask>(c->xdt), nt, P<XdOut>(c->xdo),
+                             P<DpTask>(c->tasks),
+                             (unsigned long long *)P<uint64_t>(c->cells),
+                             (unsigned long long *)(P<uint64_t>(c->cells) + 1),
+                             s));
+    if ((rc = run_dp(c, nt, false, true, 2, s)))
+        return rc;
+    HIPCHK(c, finalize_xd(P<XdTask>(c->xdt), P<XdOut>(c->xdo),
+                          P<DpOut>(c->dpout), P<uint8_t>(c->tflags), n,
+                          c->cfg.seed_cnt, c->cfg.cov_thr, c->cfg.ani_thr,
+                          d_out, s));
+    c->stats.pairs += n;
+    return PA_OK;
+}
+
+uint8_t encode_ascii(uint8_t ch)
+{
+    if (ch >= 'a' && ch <= 'z')
+        ch = (uint8_t)(ch - 'a' + 'A');
+    for (int i = 0; i < 25; ++i)
+        if ((uint8_t)kAlphabet[i] == ch)
+            return (uint8_t)i;
+    return 22;  // X
+}
+
+int upload_set(pa_ctx *c, bool rows, const uint8_t *res, const uint64_t *offs,
+               uint64_t nseq, int enc)
+{
+    int rc;
+    const uint64_t base = offs[0];
+    const uint64_t total = offs[nseq] - base;
+    std::vector<uint32_t> lens(nseq);
+    uint32_t mx = 0;
+    for (uint64_t i = 0; i < nseq; ++i) {
+        if (offs[i + 1] < offs[i])
+            return fail(c, PA_ERR_ARG, "offsets must not decrease");
+        const uint64_t L = offs[i + 1] - offs[i];
+        if (L > 65535)
+            return fail(c, PA_ERR_SEQ_LEN,
+                        "sequence " + std::to_string(i) + " has " +
+                            std::to_string(L) + " residues");
+        lens[i] = (uint32_t)L;
+        mx = std::max(mx, (uint32_t)L);
+    }
+    std::vector<uint8_t> codes(total + 64, (uint8_t)kSentinel);
+    for (uint64_t i = 0; i < total; ++i) {
+        uint8_t x = res[base + i];
+        if (enc == PA_ENC_ASCII)
+            x = encode_ascii(x);
+        else if (x >= 25)
+            return fail(c, PA_ERR_RESIDUE,
+                        "residue code " + std::to_string(x) + " at offset " +
+                            std::to_string(base + i));
+        codes[i] = x;
+    }
+    Buf &cb = rows ? c->rcodes : c->ccodes;
+    Buf &ob = rows ? c->roffs : c->coffs;
+    if ((rc = grow(c, cb, codes.size())))
+        return rc;
+    if ((rc = grow(c, ob, (nseq + 1) * sizeof(uint64_t))))
+        return rc;
+    std::vector<uint64_t> abs(nseq + 1);
+    const uint64_t dev = (uint64_t)(uintptr_t)cb.p;
+    for (uint64_t i = 0; i <= nseq; ++i)
+        abs[i] = dev + (offs[i] - base);
+    HIPCHK(c, hipMemcpyAsync(cb.p, codes.data(), codes.size(),
+                             hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(ob.p, abs.data(), abs.size() * 8,
+                             hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    if (rows) {
+        c->rlen.swap(lens);
+        c->max_rlen = mx;
+        c->have_rows = true;
+    } else {
+        c->clen.swap(lens);
+        c->max_clen = mx;
+        c->have_cols = true;
+    }
+    return PA_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char *pa_strerror(int status)
+{
+    if (status < 0 || status >= (int)(sizeof(kStatus) / sizeof(kStatus[0])))
+        return "unknown status";
+    return kStatus[status];
+}
+
+const char *pa_last_error(pa_ctx *ctx)
+{
+    return ctx ? ctx->err.c_str() : "no context";
+}
+
+int pa_config_for_cli(pa_config *cfg, int gap_open_cli, int gap_ext_cli,
+                      int klength, int xdrop, int seed_count, int gap_model)
+{
+    if (!cfg)
+        return PA_ERR_ARG;
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->mode = xdrop > 0 ? PA_MODE_XDROP : PA_MODE_SW;
+    if (gap_model == PA_GAP_AS_CALLED) {
+        // Score(gapExtend = -g, gapOpen = -e)
+        cfg->gap_open = gap_ext_cli;
+        cfg->gap_ext = gap_open_cli;
+        cfg->xdrop_gap = gap_open_cli;
+    } else if (gap_model == PA_GAP_AS_DOCUMENTED) {
+        cfg->gap_open = gap_open_cli;
+        cfg->gap_ext = gap_ext_cli;
+        cfg->xdrop_gap = gap_ext_cli;
+    } else {
+        return PA_ERR_ARG;
+    }
+    cfg->xdrop = xdrop;
+    cfg->seed_len = klength;
+    cfg->seed_cnt = seed_count;
+    cfg->ani_thr = 30;    // src/main.cpp:110
+    cfg->cov_thr = 0.7;   // src/main.cpp:109
+    cfg->device = 0;
+    cfg->matrix = nullptr;
+    return PA_OK;
+}
+
+int pa_create(const pa_config *cfg, pa_ctx **out)
+{
+    if (!cfg || !out)
+        return PA_ERR_ARG;
+    *out = nullptr;
+    if (cfg->mode != PA_MODE_SW && cfg->mode != PA_MODE_XDROP)
+        return PA_ERR_ARG;
+    // the traced int16 kernels keep 4*score + tag in int16 and use -128 as
+    // the below-everything profile entry
+    if (cfg->gap_open > -1 || cfg->gap_open < -31 || cfg->gap_ext > -1 ||
+        cfg->gap_ext < -31)
+        return PA_ERR_ARG;
+    if (cfg->mode == PA_MODE_XDROP &&
+        (cfg->xdrop <= 0 || cfg->xdrop_gap > -1 || cfg->seed_len < 1 ||
+         cfg->seed_cnt < 1 || cfg->seed_cnt > 2))
+        return PA_ERR_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= cfg->device ||
+        cfg->device < 0)
+        return PA_ERR_NODEV;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, cfg->device) != hipSuccess ||
+        std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return PA_ERR_NODEV;
+    pa_ctx *c = new pa_ctx();
+    c->cfg = *cfg;
+    c->device = cfg->device;
+    std::memset(c->mat, 0, sizeof(c->mat));
+    for (int a = 0; a < 25; ++a)
+        for (int b = 0; b < 25; ++b) {
+            const int8_t v = cfg->matrix ? cfg->matrix[a * 32 + b]
+                                         : kBlosum62[a * 25 + b];
+            if (v < -32 || v > 31) {
+                delete c;
+                return PA_ERR_ARG;
+            }
+            c->mat[a * 32 + b] = v;
+            c->maxs = std::max<int>(c->maxs, v);
+        }
+    c->cfg.matrix = nullptr;
+    if (hipSetDevice(c->device) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return PA_ERR_HIP;
+    }
+    if (grow(c, c->dmat, sizeof(c->mat)) ||
+        hipMemcpy(c->dmat.p, c->mat, sizeof(c->mat), hipMemcpyHostToDevice) !=
+            hipSuccess) {
+        pa_destroy(c);
+        return PA_ERR_NOMEM;
+    }
+    size_t fr = 0, tot = 0;
+    hipMemGetInfo(&fr, &tot);
+    c->trace_cap = std::min<size_t>((size_t)8 << 30, fr / 4);
+    if (const char *e = std::getenv("PA_TRACE_BYTES"))
+        c->trace_cap = std::max<size_t>(std::strtoull(e, nullptr, 10), 1 << 20);
+    *out = c;
+    return PA_OK;
+}
+
+void pa_destroy(pa_ctx *c)
+{
+    if (!c)
+        return;
+    hipSetDevice(c->device);
+    if (c->st)
+        hipStreamSynchronize(c->st);
+    resolve_timers(c);
+    if (c->shared_cols) {
+        c->ccodes = Buf();
+        c->coffs = Buf();
+    }
+    for (Buf *b : {&c->rcodes, &c->ccodes, &c->roffs, &c->coffs, &c->pairs,
+                   &c->results, &c->scores, &c->tasks, &c->dpout, &c->tflags,
+                   &c->sizes, &c->toffs, &c->lists, &c->sat, &c->ctr,
+                   &c->trace, &c->bnd, &c->xdt, &c->xdo, &c->xds, &c->ovf,
+                   &c->cub, &c->cells, &c->dmat, &c->ext})
+        release(*b);
+    for (auto e : c->free_events)
+        hipEventDestroy(e);
+    if (c->st)
+        hipStreamDestroy(c->st);
+    delete c;
+}
+
+int pa_load_seqs(pa_ctx *c, int side, const uint8_t *residues,
+                 const uint64_t *offsets, uint64_t nseq, int encoding)
+{
+    if (!c || !offsets || (nseq && !residues))
+        return PA_ERR_ARG;
+    if (side < PA_SIDE_ROW || side > PA_SIDE_BOTH ||
+        (encoding != PA_ENC_CODES && encoding != PA_ENC_ASCII))
+        return fail(c, PA_ERR_ARG, "bad side or encoding");
+    hipSetDevice(c->device);
+    int rc;
+    if (side == PA_SIDE_ROW || side == PA_SIDE_BOTH) {
+        if (c->shared_cols) {
+            // the column set aliased the row buffers; detach it first
+            c->ccodes = Buf();
+            c->coffs = Buf();
+            c->have_cols = false;
+            c->shared_cols = false;
+        }
+        if ((rc = upload_set(c, true, residues, offsets, nseq, encoding)))
+            return rc;
+    }
+    if (side == PA_SIDE_COL) {
+        if (c->shared_cols) {
+            c->ccodes = Buf();
+            c->coffs = Buf();
+            c->shared_cols = false;
+        }
+        if ((rc = upload_set(c, false, residues, offsets, nseq, encoding)))
+            return rc;
+    }
+    if (side == PA_SIDE_BOTH) {
+        if (!c->shared_cols) {
+            release(c->ccodes);
+            release(c->coffs);
+        }
+        c->ccodes = c->rcodes;
+        c->coffs = c->roffs;
+        c->clen = c->rlen;
+        c->max_clen = c->max_rlen;
+        c->have_cols = true;
+        c->shared_cols = true;
+    }
+    return PA_OK;
+}
+
+int pa_align_dev(pa_ctx *c, const pa_pair *d_pairs, uint64_t n,
+                 pa_result *d_out, void *stream)
+{
+    int rc;
+    if ((rc = check_ready(c)))
+        return rc;
+    if (n == 0)
+        return PA_OK;
+    if (!d_pairs || !d_out)
+        return fail(c, PA_ERR_ARG, "null pointer");
+    if (n > 0x7fffffffull / 2)
+        return fail(c, PA_ERR_LIMIT, "batch larger than 2^30 pairs");
+    hipSetDevice(c->device);
+    hipStream_t s = pick(c, stream);
+    return c->cfg.mode == PA_MODE_SW ? align_sw_dev(c, d_pairs, n, d_out, s)
+                                     : align_xd_dev(c, d_pairs, n, d_out, s);
+}
+
+int pa_align(pa_ctx *c, const pa_pair *pairs, uint64_t n, pa_result *out)
+{
+    int rc;
+    if ((rc = check_ready(c)))
+        return rc;
+    if (n == 0)
+        return PA_OK;
+    if (!pairs || !out)
+        return fail(c, PA_ERR_ARG, "null pointer");
+    if ((rc = validate_pairs_host(c, pairs, n)))
+        return rc;
+    hipSetDevice(c->device);
+    if ((rc = grow(c, c->pairs, n * sizeof(pa_pair))))
+        return rc;
+    if ((rc = grow(c, c->results, n * sizeof(pa_result))))
+        return rc;
+    HIPCHK(c, hipMemcpyAsync(c->pairs.p, pairs, n * sizeof(pa_pair),
+                             hipMemcpyHostToDevice, c->st));
+    if ((rc = pa_align_dev(c, P<pa_pair>(c->pairs), n, P<pa_result>(c->results),
+                           nullptr)))
+        return rc;
+    HIPCHK(c, hipMemcpyAsync(out, c->results.p, n * sizeof(pa_result),
+                             hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    resolve_timers(c);
+    return PA_OK;
+}
+
+int pa_sw_scores_dev(pa_ctx *c, const pa_pair *d_pairs, uint64_t n,
+                     pa_score *d_out, void *stream)
+{
+    int rc;
+    if ((rc = check_ready(c)))
+        return rc;
+    if (n == 0)
+        return PA_OK;
+    if (!d_pairs || !d_out)
+        return fail(c, PA_ERR_ARG, "null pointer");
+    hipSetDevice(c->device);
+    hipStream_t s = pick(c, stream);
+    if ((rc = ensure_common(c, n)))
+        return rc;
+    Scope tot(c, 4, s);
+    HIPCHK(c, build_sw_tasks(d_pairs, n, P<uint64_t>(c->roffs),
+                             P<uint64_t>(c->coffs), P<DpTask>(c->tasks),
+                             (unsigned long long *)P<uint64_t>(c->cells), s));
+    if ((rc = run_dp(c, n, true, false, 0, s)))
+        return rc;
+    HIPCHK(c, scores_out(P<DpOut>(c->dpout), n, d_out, s));
+    c->stats.pairs += n;
+    return PA_OK;
+}
+
+int pa_sw_scores(pa_ctx *c, const pa_pair *pairs, uint64_t n, pa_score *out)
+{
+    int rc;
+    if ((rc = check_ready(c)))
+        return rc;
+    if (n == 0)
+        return PA_OK;
+    if (!pairs || !out)
+        return fail(c, PA_ERR_ARG, "null pointer");
+    for (uint64_t i = 0; i < n; ++i)
+        if (pairs[i].r >= c->rlen.size() || pairs[i].c >= c->clen.size())
+            return fail(c, PA_ERR_INDEX,
+                        "pair " + std::to_string(i) + " outside the loaded sets");
+    hipSetDevice(c->device);
+    if ((rc = grow(c, c->pairs, n * sizeof(pa_pair))))
+        return rc;
+    if ((rc = grow(c, c->scores, n * sizeof(pa_score))))
+        return rc;
+    HIPCHK(c, hipMemcpyAsync(c->pairs.p, pairs, n * sizeof(pa_pair),
+                             hipMemcpyHostToDevice, c->st));
+    if ((rc = pa_sw_scores_dev(c, P<pa_pair>(c->pairs), n,
+                               P<pa_score>(c->scores), nullptr)))
+        return rc;
+    HIPCHK(c, hipMemcpyAsync(out, c->scores.p, n * sizeof(pa_score),
+                             hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    resolve_timers(c);
+    return PA_OK;
+}
+
+int pa_xdrop_extents(pa_ctx *c, const pa_pair *pairs, uint64_t n, int seed,
+                     pa_extent *out)
+{
+    int rc;
+    if ((rc = check_ready(c)))
+        return rc;
+    if (c->cfg.mode != PA_MODE_XDROP)
+        return fail(c, PA_ERR_ARG, "context is not in X-drop mode");
+    if (seed < 0 || seed >= c->cfg.seed_cnt)
+        return fail(c, PA_ERR_ARG, "seed index");
+    if (n == 0)
+        return PA_OK;
+    if (!pairs || !out)
+        return fail(c, PA_ERR_ARG, "null pointer");
+    if ((rc = validate_pairs_host(c, pairs, n)))
+        return rc;
+    hipSetDevice(c->device);
+    if ((rc = ensure_common(c, n * 2)))
+        return rc;
+    if ((rc = grow(c, c->pairs, n * sizeof(pa_pair))))
+        return rc;
+    if ((rc = grow(c, c->ext, n * sizeof(pa_extent))))
+        return rc;
+    HIPCHK(c, hipMemcpyAsync(c->pairs.p, pairs, n * sizeof(pa_pair),
+                             hipMemcpyHostToDevice, c->st));
+    if ((rc = run_xdrop(c, P<pa_pair>(c->pairs), n, c->st)))
+        return rc;
+    HIPCHK(c, extents_out(P<XdOut>(c->xdo), n, seed, P<pa_extent>(c->ext),
+                          c->st));
+    HIPCHK(c, hipMemcpyAsync(out, c->ext.p, n * sizeof(pa_extent),
+                             hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    resolve_timers(c);
+    return PA_OK;
+}
+
+int pa_kmer_prune(const pa_pair *pairs, uint64_t n, int ckthr, float mosthr,
+                  uint8_t *keep, uint64_t *kept)
+{
+    if ((n && (!pairs || !keep)))
+        return PA_ERR_ARG;
+    uint64_t k = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        // DistPWRunner.cpp:48-54: prune if count <= ckthr || score <= mosthr
+        const bool prune = (int)pairs[i].count <= ckthr ||
+                           (float)pairs[i].kscore <= mosthr;
+        keep[i] = prune ? 0 : 1;
+        k += !prune;
+    }
+    if (kept)
+        *kept = k;
+    return PA_OK;
+}
+
+int pa_device_alloc(pa_ctx *c, size_t bytes, void **ptr)
+{
+    if (!c || !ptr)
+        return PA_ERR_ARG;
+    hipSetDevice(c->device);
+    if (hipMalloc(ptr, bytes ? bytes : 1) != hipSuccess)
+        return fail(c, PA_ERR_NOMEM, "pa_device_alloc");
+    return PA_OK;
+}
+
+int pa_device_free(pa_ctx *c, void *ptr)
+{
+    if (!c)
+        return PA_ERR_ARG;
+    hipSetDevice(c->device);
+    HIPCHK(c, hipFree(ptr));
+    return PA_OK;
+}
+
+int pa_memcpy_h2d(pa_ctx *c, void *dst, const void *src, size_t bytes)
+{
+    if (!c)
+        return PA_ERR_ARG;
+    hipSetDevice(c->device);
+    HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return PA_OK;
+}
+
+int pa_memcpy_d2h(pa_ctx *c, void *dst, const void *src, size_t bytes)
+{
+    if (!c)
+        return PA_ERR_ARG;
+    hipSetDevice(c->device);
+    HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return PA_OK;
+}
+
+int pa_synchronize(pa_ctx *c)
+{
+    if (!c)
+        return PA_ERR_ARG;
+    hipSetDevice(c->device);
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    resolve_timers(c);
+    return PA_OK;
+}
+
+int pa_get_stats(pa_ctx *c, pa_stats *out)
+{
+    if (!c || !out)
+        return PA_ERR_ARG;
+    hipSetDevice(c->device);
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    resolve_timers(c);
+    if (c->cells.p) {
+        uint64_t cells[2] = {0, 0};
+        HIPCHK(c, hipMemcpy(cells, c->cells.p, 16, hipMemcpyDeviceToHost));
+        c->stats.dp_cells = cells[0];
+        c->stats.xdrop_cells = cells[1];
+    }
+    *out = c->stats;
+    return PA_OK;
+}
+
+int pa_reset_stats(pa_ctx *c)
+{
+    if (!c)
+        return PA_ERR_ARG;
+    hipSetDevice(c->device);
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    resolve_timers(c);
+    c->stats = pa_stats{};
+    if (c->cells.p)
+        HIPCHK(c, hipMemset(c->cells.p, 0, 32));
+    return PA_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,118 @@
+// pa_internal.h -- structures shared by the HIP kernels and the host side of
+// libpastis_hip.so.  Not part of the public ABI (include/pastis_hip.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/pastis_hip.h"
+
+namespace pa {
+
+// Residue codes 0..24 are real letters; code 25 is the sentinel used for the
+// DP's column 0 and for columns past the end of the streamed sequence.  Its
+// profile entries are -128, strictly below any real score.
+constexpr int kAlpha = 25;
+constexpr int kSentinel = 25;
+constexpr int kProfChars = 26;
+
+// One DP task: the row (V, lanes) and column (H, streamed) sequence spans.
+struct DpTask {
+    uint64_t q_off;   // first residue of the row span in the sequence store
+    uint64_t t_off;   // first residue of the column span
+    uint32_t qlen;    // rows
+    uint32_t tlen;    // columns
+    uint32_t id;      // output slot
+    uint32_t flags;
+};
+static_assert(sizeof(DpTask) == 32, "DpTask layout");
+
+// Output of a DP task: score and end cell (1-based DP coordinates, i.e. the
+// exclusive 0-based end), plus traceback statistics when traced.
+struct DpOut {
+    int32_t score;
+    int32_t q_end, t_end;     // rows / columns consumed at the end cell
+    int32_t q_beg, t_beg;     // traceback start (filled by the traceback)
+    int32_t matches, cols, gap_opens;
+};
+static_assert(sizeof(DpOut) == 32, "DpOut layout");
+
+// Task flags
+constexpr uint32_t kTaskSat = 1u;  // int16 saturated, needs the int32 re-run
+
+// Arguments of the wavefront DP kernels.
+struct DpArgs {
+    const uint8_t *codes;      // sequence store
+    const DpTask *tasks;
+    const uint32_t *list;      // task indices of this launch
+    const uint32_t *count;     // device: number of entries in list
+    uint32_t *queue;           // device: work-queue head (zeroed before launch)
+    const int8_t *mat;         // 32x32 scores
+    DpOut *out;
+    uint8_t *trace;            // trace store (traced kernels)
+    const uint64_t *trace_off; // per task byte offset into trace
+    int32_t *bnd;              // per-workgroup boundary rows (multi-pass)
+    uint32_t bnd_stride;       // int32 entries per workgroup
+    uint32_t *sat_list;        // tasks that saturated (int16 kernels)
+    uint32_t *sat_count;
+    int32_t gap_open, gap_ext;
+};
+
+// Arguments of the traceback kernel.
+struct TbArgs {
+    const uint8_t *codes;
+    const DpTask *tasks;
+    const uint32_t *list;
+    const uint32_t *count;
+    const uint8_t *trace;
+    const uint64_t *trace_off;
+    const uint8_t *rows_per_lane; // R of the kernel that traced each task
+    DpOut *out;
+    int local;
+};
+
+// Bytes of trace a task needs when traced with R rows per lane.
+__host__ __device__ inline uint64_t trace_bytes(uint32_t qlen, uint32_t tlen,
+                                                int R)
+{
+    uint32_t rows = 128u * (uint32_t)R;
+    uint32_t passes = (qlen + rows - 1) / rows;
+    return (uint64_t)passes * (uint64_t)(tlen + 130u) * 64u * (uint64_t)R;
+}
+
+// Rows per lane chosen for a query length (256, 512 or 1024 rows per pass).
+__host__ __device__ inline int rows_per_lane(uint32_t qlen)
+{
+    return qlen <= 256 ? 2 : (qlen <= 512 ? 4 : 8);
+}
+
+// X-drop extension task: one seed of one pair.
+struct XdTask {
+    uint64_t h_off, v_off;     // column (H) and row (V) sequence starts
+    uint32_t nh, nv;           // lengths
+    uint32_t beg_h, beg_v;     // seed start
+    uint32_t id;               // output slot
+    uint32_t pad;
+};
+static_assert(sizeof(XdTask) == 40, "XdTask layout");
+
+struct XdOut {
+    int32_t beg_h, beg_v, end_h, end_v;
+    int32_t score_left, score_right;
+    uint32_t cells;
+    uint32_t flags;            // bit 0: band exceeded the scratch capacity
+};
+static_assert(sizeof(XdOut) == 32, "XdOut layout");
+
+// Launchers (k_dp.hip, k_xdrop.hip, k_misc.hip).
+hipError_t launch_dp(int R, bool local, bool trace, bool i16,
+                     const DpArgs &a, int grid, hipStream_t s);
+hipError_t launch_traceback(const TbArgs &a, int grid, hipStream_t s);
+hipError_t launch_xdrop_k(const XdTask *tasks, const uint32_t *list,
+                          const uint32_t *count, uint32_t *queue,
+                          const uint8_t *codes, const int8_t *mat, int gap,
+                          int xdrop, int32_t *scratch, uint32_t cap,
+                          XdOut *out, uint32_t *ovf_list, uint32_t *ovf_count,
+                          int seed_len, int grid, hipStream_t s);
+
+}  // namespace pa

@@ -1,0 +1,98 @@
+"""HIP seed-and-extend X-drop path against the CPU oracle, bit-exact:
+extension extents per seed (pa_xdrop_extents) and the full --sxa operator
+(global alignment of the extended infixes, best seed, predicate)."""
+import numpy as np
+import pytest
+
+from helpers import (GAP_MODELS, oracle_params, pairs_of, rand_seq, seqset,
+                     small_config2)
+from oracle import oracle as orc
+from pastis_amd import _lib
+from pastis_amd.aligner import Aligner, config_for_cli
+
+pytestmark = pytest.mark.gpu
+
+
+def make_aligner(model, seed_cnt=2):
+    return Aligner(config_for_cli(-11, -2, 6, 49, seed_cnt, model))
+
+
+def check_extents(al, codes, offs, pairs, model, seed_cnt=2):
+    g = GAP_MODELS[model]
+    for s in range(seed_cnt):
+        got = al.xdrop_extents(pairs, s)
+        for k in range(len(pairs)):
+            r, c = int(pairs["r"][k]), int(pairs["c"][k])
+            v = codes[offs[r]:offs[r + 1]]
+            h = codes[offs[c]:offs[c + 1]]
+            ro, co = int(pairs["seed"][k][2 * s]), int(pairs["seed"][k][2 * s + 1])
+            x = orc.xdrop_extend(h, v, co, ro, 6, g["xdrop_gap"], 49)
+            assert (got["r_beg"][k], got["r_end"][k], got["c_beg"][k],
+                    got["c_end"][k]) == (x["beg_v"], x["end_v"], x["beg_h"],
+                                         x["end_h"]), (k, s)
+
+
+def check_align(al, codes, offs, pairs, model, seed_cnt=2):
+    p = oracle_params(model, seed_cnt=seed_cnt)
+    got = al.align(pairs)
+    ref = orc.sxa_batch(codes, offs, pairs["r"], pairs["c"], pairs["seed"], p)
+    for gf, rf in [("score", "score"), ("matches", "matches"),
+                   ("cols", "cols"), ("gap_opens", "gap_opens"),
+                   ("seed", "seed")]:
+        bad = np.nonzero(got[gf].astype(np.int64) != ref[rf].astype(np.int64))[0]
+        assert bad.size == 0, (gf, int(bad[0]), got[bad[0]], ref[bad[0]])
+    for gf, rf in [("r_beg", "v_beg"), ("r_end", "v_end"), ("c_beg", "h_beg"),
+                   ("c_end", "h_end")]:
+        assert (got[gf].astype(np.int64) == ref[rf].astype(np.int64)).all(), gf
+    assert (got["identity"] == ref["identity"]).all()
+    assert (got["score_aln"] == ref["score_aln"]).all()
+    assert ((got["flags"] & _lib.PA_FLAG_PASS).astype(bool)
+            == ref["passed"].astype(bool)).all()
+    return got
+
+
+@pytest.mark.parametrize("model", ["as_called", "as_documented"])
+def test_xdrop_extents_synthetic(model):
+    w = small_config2(n_pairs=600)
+    al = make_aligner(model)
+    al.load_seqs(w.codes, w.offsets)
+    check_extents(al, w.codes, w.offsets, w.pairs, model)
+
+
+@pytest.mark.parametrize("model", ["as_called", "as_documented"])
+def test_sxa_operator_synthetic(model):
+    w = small_config2()
+    al = make_aligner(model)
+    al.load_seqs(w.codes, w.offsets)
+    got = check_align(al, w.codes, w.offsets, w.pairs, model)
+    assert (got["flags"] & _lib.PA_FLAG_PASS).any()
+
+
+def test_sxa_single_seed_and_edge_seeds():
+    rng = np.random.default_rng(4)
+    a = rand_seq(rng, 400)
+    b = a.copy()
+    b[::9] = rand_seq(rng, len(b[::9]))
+    seqs = [a, b, rand_seq(rng, 6), rand_seq(rng, 1500), a[:200].copy()]
+    codes, offs = seqset(seqs)
+    # seeds at the very start, the very end and (0,0) second seeds
+    rc = [(0, 1), (1, 0), (0, 4), (4, 0), (2, 2), (3, 0), (0, 3)]
+    seeds = [[0, 0, 394, 394], [394, 394, 0, 0], [10, 10, 0, 0],
+             [194, 194, 0, 0], [0, 0, 0, 0], [700, 100, 0, 0],
+             [100, 700, 0, 0]]
+    pairs = pairs_of(rc, seeds)
+    for model in GAP_MODELS:
+        for sc in (1, 2):
+            al = make_aligner(model, seed_cnt=sc)
+            al.load_seqs(codes, offs)
+            check_extents(al, codes, offs, pairs, model, seed_cnt=sc)
+            check_align(al, codes, offs, pairs, model, seed_cnt=sc)
+
+
+def test_seed_out_of_range_is_an_error():
+    rng = np.random.default_rng(6)
+    codes, offs = seqset([rand_seq(rng, 20), rand_seq(rng, 30)])
+    al = make_aligner("as_called")
+    al.load_seqs(codes, offs)
+    with pytest.raises(_lib.PaError):
+        al.align(pairs_of([(0, 1)], [[15, 0, 0, 0]]))
