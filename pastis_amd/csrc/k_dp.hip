@@ -265,7 +265,7 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
     constexpr int ROWS = 128 * R;
     constexpr int SH = TRACE ? 2 : 0;
     constexpr int NEG = TRACE ? O::kNegTraced : O::kNeg;
-    extern __shared__ uint32_t prof32[];
+    HIP_DYNAMIC_SHARED(uint32_t, prof32)
     const uint8_t *prof = (const uint8_t *)prof32;
     const int lane = threadIdx.x;
     const uint32_t count = *a.count;
@@ -277,9 +277,10 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
     int32_t *bnd = a.bnd + (size_t)blockIdx.x * a.bnd_stride;
 
     for (;;) {
-        uint32_t k = 0;
-        if (lane == 0)
-            k = atomicAdd(a.queue, 1u);
+        // Every lane takes part in the queue atomic (only lane 0 adds): a
+        // lane-0-only branch ahead of the readfirstlane lets the compiler
+        // rotate the loop so that the other lanes re-enter it without lane 0.
+        uint32_t k = atomicAdd(a.queue, lane == 0 ? 1u : 0u);
         k = __builtin_amdgcn_readfirstlane(k);
         if (k >= count)
             break;
@@ -354,11 +355,9 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
             }
             V upH = O::shuffle_up(outH), upF = O::shuffle_up(outF);
             V dg = O::splat(NEG);
-            if (lane == 0) {
-                upH = O::set_lo(upH, bh1);
-                upF = O::set_lo(upF, bf1);
-                dg = O::set_lo(dg, bh0);
-            }
+            upH = lane == 0 ? O::set_lo(upH, bh1) : upH;
+            upF = lane == 0 ? O::set_lo(upF, bf1) : upF;
+            dg = lane == 0 ? O::set_lo(dg, bh0) : dg;
             int c_lo = kSentinel, c_hi = kSentinel;
             const int last_step = tlen + 2 * (nact - 1) + 1;
             // cell (qlen, tlen) for the global score
@@ -372,16 +371,17 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
             for (int t = 1; t <= last_step; ++t) {
                 // column residues: lane 0 reads, the others take lane-1's
                 // residue of the previous step (two columns back for them)
+                // (no lane-dependent branches around the DPP moves: lane 0's
+                // inputs are selected, every lane loads the same address)
                 {
-                    int nc = __builtin_amdgcn_update_dpp(0, c_hi, 0x138, 0xf,
-                                                         0xf, false);
+                    const int nc = __builtin_amdgcn_update_dpp(
+                        0, c_hi, 0x138, 0xf, 0xf, false);
+                    const int c0 = t <= tlen ? (int)ts[t - 1] : kSentinel;
                     c_hi = c_lo;
-                    if (lane == 0)
-                        nc = t <= tlen ? (int)ts[t - 1] : kSentinel;
-                    c_lo = nc;
+                    c_lo = lane == 0 ? c0 : nc;
                 }
                 int nbh = 0, nbf = NEG;  // boundary for lane 0 at step t+1
-                if (lane == 0) {
+                {
                     const int jn = t + 1;
                     if (p == 0) {
                         nbh = LOCAL ? 0 : open + (jn - 1) * gmax;
@@ -482,10 +482,8 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
                 dg = upH;
                 upH = O::shuffle_up(outH);
                 upF = O::shuffle_up(outF);
-                if (lane == 0) {
-                    upH = O::set_lo(upH, nbh);
-                    upF = O::set_lo(upF, nbf);
-                }
+                upH = lane == 0 ? O::set_lo(upH, nbh) : upH;
+                upF = lane == 0 ? O::set_lo(upF, nbf) : upF;
             }
         }
 
@@ -523,10 +521,12 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
                 o.t_end = 0xffff - (int)((best >> 24) & 0xffff);
                 o.q_end = 0xffffff - (int)(best & 0xffffff) + 1;
             }
-            if (I16 && lane == 0) {
+            if (I16) {
                 const int lim = TRACE ? 8190 : 32766;
-                if (sc >= lim) {
-                    uint32_t pos = atomicAdd(a.sat_count, 1u);
+                if (sc >= lim) {  // wave-uniform
+                    uint32_t pos =
+                        atomicAdd(a.sat_count, lane == 0 ? 1u : 0u);
+                    pos = __builtin_amdgcn_readfirstlane(pos);
                     a.sat_list[pos] = tid;
                 }
             }
@@ -539,11 +539,11 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
             o.q_end = qlen;
             o.t_end = tlen;
         }
-        if (lane == 0) {
-            o.q_beg = o.q_end;
-            o.t_beg = o.t_end;
-            a.out[tid] = o;
-        }
+        // all lanes hold the same result; every lane stores it (same
+        // address, same bytes) so no lane-0 branch closes the task loop
+        o.q_beg = o.q_end;
+        o.t_beg = o.t_end;
+        a.out[tid] = o;
     }
 }
 

@@ -138,9 +138,10 @@ __global__ void k_bin(const DpTask *__restrict__ tasks, uint32_t a, uint32_t b,
         if (!m)
             continue;
         const int leader = __ffsll((unsigned long long)m) - 1;
-        uint32_t base = 0;
-        if (lane == leader)
-            base = atomicAdd(&counts[bb], (uint32_t)__popcll(m));
+        // every lane joins the atomic (only the leader adds) so that no
+        // divergent branch precedes the shuffle
+        uint32_t base = atomicAdd(&counts[bb],
+                                  lane == leader ? (uint32_t)__popcll(m) : 0u);
         base = __shfl(base, leader);
         if (bin == bb) {
             const uint32_t rank =

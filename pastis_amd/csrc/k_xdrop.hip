@@ -157,15 +157,14 @@ __global__ __launch_bounds__(256) void k_xdrop(
     int32_t *buf = scratch + (size_t)(blockIdx.x * blockDim.x + threadIdx.x) *
                                  (size_t)(3 * cap);
     for (;;) {
-        uint32_t base = 0;
-        if (lane == 0)
-            base = atomicAdd(queue, 64u);
+        // all lanes join the atomic (lane 0 adds 64) and the loop body has
+        // no divergent continue, so the wave re-enters the loop together
+        uint32_t base = atomicAdd(queue, lane == 0 ? 64u : 0u);
         base = __builtin_amdgcn_readfirstlane(base);
         if (base >= n)
             break;
         const uint32_t k = base + (uint32_t)lane;
-        if (k >= n)
-            continue;
+        if (k < n) {
         const uint32_t tid = list ? list[k] : k;
         const XdTask T = tasks[tid];
         const uint8_t *h = (const uint8_t *)(uintptr_t)T.h_off;
@@ -184,7 +183,7 @@ __global__ __launch_bounds__(256) void k_xdrop(
                         &cells);
         if (L.ovf || Rr.ovf) {
             o.flags = 1;
-            const uint32_t pos = atomicAdd(ovf_count, 1u);
+            const uint32_t pos = atomicAdd(ovf_count, 1u);  // no wave op follows
             ovf_list[pos] = tid;
         } else {
             if (L.upd) {
@@ -204,6 +203,7 @@ __global__ __launch_bounds__(256) void k_xdrop(
             o.cells = cells;
         }
         out[T.id] = o;
+        }
     }
 }
 

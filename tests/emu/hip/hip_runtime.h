@@ -1,0 +1,184 @@
+// CPU emulation of the HIP subset used by pastis_amd/csrc -- TEST ONLY.
+//
+// Compiles the library's kernel and host sources for the CPU so that hangs,
+// out-of-bounds accesses (under AddressSanitizer) and parity bugs can be
+// found without a GPU.  Each workgroup runs as blockDim.x std::threads; the
+// wave64 collectives the kernels use (DPP wave_shr:1, readfirstlane, shfl,
+// ballot) exchange values through per-wave slots between two barriers, which
+// matches the hardware as long as every lane of the wave reaches the
+// collective (true for these kernels: collectives sit outside divergent
+// code).  Workgroups run one after another.  Never linked into the product.
+#pragma once
+
+#include <atomic>
+#include <barrier>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <thread>
+#include <vector>
+
+#define __global__
+#define __device__
+#define __host__
+#define __launch_bounds__(...)
+#define __shared__ static
+#define __forceinline__ inline
+#define HIP_DYNAMIC_SHARED(type, var) type *var = (type *)emu::dyn_shared();
+
+struct dim3 {
+    unsigned x, y, z;
+    dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
+};
+struct uint2 {
+    unsigned x, y;
+};
+inline uint2 make_uint2(unsigned a, unsigned b) { return {a, b}; }
+
+typedef int hipError_t;
+constexpr hipError_t hipSuccess = 0;
+constexpr hipError_t hipErrorInvalidValue = 1;
+constexpr hipError_t hipErrorOutOfMemory = 2;
+typedef struct emu_stream *hipStream_t;
+typedef struct emu_event *hipEvent_t;
+enum hipMemcpyKind {
+    hipMemcpyHostToDevice,
+    hipMemcpyDeviceToHost,
+    hipMemcpyDeviceToDevice,
+    hipMemcpyHostToHost
+};
+constexpr unsigned hipStreamNonBlocking = 1;
+struct hipDeviceProp_t {
+    char gcnArchName[256];
+};
+
+namespace emu {
+extern thread_local dim3 tIdx, bIdx, bDim, gDim;
+void *dyn_shared();
+uint64_t wave_exchange(uint64_t v, int src_lane, bool xor_mode);
+uint64_t wave_ballot(bool p);
+void block_sync();
+void launch_impl(dim3 g, dim3 b, size_t shmem, void (*body)(void *),
+                 void *arg);
+
+template <class F> void trampoline(void *p) { (*(F *)p)(); }
+
+template <class K, class... A>
+void launch(K kernel, dim3 g, dim3 b, size_t shmem, A... args)
+{
+    auto f = [=]() { kernel(args...); };
+    launch_impl(g, b, shmem, &trampoline<decltype(f)>, (void *)&f);
+}
+}  // namespace emu
+
+#define threadIdx emu::tIdx
+#define blockIdx emu::bIdx
+#define blockDim emu::bDim
+#define gridDim emu::gDim
+
+#define hipLaunchKernelGGL(kernel, grid, block, shmem, stream, ...) \
+    emu::launch(kernel, dim3(grid), dim3(block), (size_t)(shmem), __VA_ARGS__)
+
+inline void __syncthreads() { emu::block_sync(); }
+inline void __threadfence() { std::atomic_thread_fence(std::memory_order_seq_cst); }
+inline void __threadfence_block() { __threadfence(); }
+
+template <class T> T atomicAdd(T *p, T v)
+{
+    return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST);
+}
+
+template <class T> inline T emu_bits_in(uint64_t x)
+{
+    T r;
+    std::memcpy(&r, &x, sizeof(T));
+    return r;
+}
+template <class T> inline uint64_t emu_bits_out(T v)
+{
+    uint64_t x = 0;
+    std::memcpy(&x, &v, sizeof(T));
+    return x;
+}
+template <class T> T __shfl(T v, int src)
+{
+    return emu_bits_in<T>(emu::wave_exchange(emu_bits_out(v), src & 63, false));
+}
+template <class T> T __shfl_xor(T v, int m)
+{
+    return emu_bits_in<T>(emu::wave_exchange(emu_bits_out(v), m, true));
+}
+inline uint64_t __ballot(bool p) { return emu::wave_ballot(p); }
+inline int __ffsll(unsigned long long x) { return __builtin_ffsll((long long)x); }
+inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
+inline float __int_as_float(int x)
+{
+    float f;
+    std::memcpy(&f, &x, 4);
+    return f;
+}
+
+// wave_shr:1 (control 0x138) is the only DPP pattern the kernels use.
+inline int emu_update_dpp(int old, int src, int ctrl, int, int, bool)
+{
+    if (ctrl != 0x138)
+        std::abort();
+    const int lane = (int)(emu::tIdx.x & 63);
+    uint64_t v = emu::wave_exchange((uint32_t)src, lane - 1, false);
+    return lane == 0 ? old : (int)(uint32_t)v;
+}
+#define __builtin_amdgcn_update_dpp emu_update_dpp
+inline unsigned emu_readfirstlane(unsigned v)
+{
+    return (unsigned)emu::wave_exchange(v, 0, false);
+}
+#define __builtin_amdgcn_readfirstlane emu_readfirstlane
+inline unsigned emu_perm(unsigned s0, unsigned s1, unsigned sel)
+{
+    const uint64_t src = ((uint64_t)s0 << 32) | s1;
+    unsigned out = 0;
+    for (int i = 0; i < 4; ++i) {
+        const unsigned s = (sel >> (8 * i)) & 0xff;
+        unsigned b;
+        if (s < 8)
+            b = (unsigned)(src >> (8 * s)) & 0xff;
+        else if (s < 12)
+            b = ((src >> (16 * (s - 8) + 15)) & 1) ? 0xff : 0;
+        else if (s == 12)
+            b = 0;
+        else
+            b = 0xff;
+        out |= b << (8 * i);
+    }
+    return out;
+}
+#define __builtin_amdgcn_perm emu_perm
+inline unsigned emu_alignbit(unsigned a, unsigned b, unsigned s)
+{
+    return (unsigned)((((uint64_t)a << 32) | b) >> (s & 31));
+}
+#define __builtin_amdgcn_alignbit emu_alignbit
+
+// runtime
+hipError_t hipMalloc(void **p, size_t n);
+hipError_t hipFree(void *p);
+hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind);
+hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, hipMemcpyKind,
+                          hipStream_t);
+hipError_t hipMemsetAsync(void *d, int v, size_t n, hipStream_t);
+hipError_t hipMemset(void *d, int v, size_t n);
+hipError_t hipStreamSynchronize(hipStream_t);
+hipError_t hipStreamCreateWithFlags(hipStream_t *, unsigned);
+hipError_t hipStreamDestroy(hipStream_t);
+hipError_t hipEventCreate(hipEvent_t *);
+hipError_t hipEventRecord(hipEvent_t, hipStream_t);
+hipError_t hipEventSynchronize(hipEvent_t);
+hipError_t hipEventElapsedTime(float *, hipEvent_t, hipEvent_t);
+hipError_t hipEventDestroy(hipEvent_t);
+hipError_t hipGetDeviceCount(int *);
+hipError_t hipGetDeviceProperties(hipDeviceProp_t *, int);
+hipError_t hipSetDevice(int);
+hipError_t hipMemGetInfo(size_t *, size_t *);
+hipError_t hipGetLastError();
+const char *hipGetErrorString(hipError_t);
