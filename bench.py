@@ -1,0 +1,246 @@
+"""Benchmark of the batched pairwise aligner on one MI355X.
+
+A step is one pass of the aligner operator (PWAlign::aln_batch replacement,
+pa_align_dev) over one batch of synthetic candidate pairs whose sequences and
+pairs are already resident in HBM.  The default workload is the seed-and-
+extend X-drop configuration: 20,000 proteins with log-normal lengths (median
+250, sigma 0.6, clipped to 30..3000), 5,000,000 candidate pairs with up to two
+shared 6-mer seeds, 30 % planted homologs at 30..95 % identity, BLOSUM62,
+X = 49.  Each step runs the X-drop extension of both seeds, the global
+alignment of the extended infixes with traceback statistics, the best-seed
+choice and the coverage/identity predicate.
+
+Prints one JSON line.  --workload sw runs the plain local Smith-Waterman
+configuration (2,000 proteins U[50,500], 100,000 pairs, score and end cell).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("GCUPS + aligned pairs/s on 1 MI355X; learned-filter vs plain at "
+          ">=99.5% recall")
+# packed int16 VALU peak: 256 CU x 64 lanes x 2 halves x 2.4 GHz
+PEAK_I16_TOPS = 78.6
+PEAK_I32_TOPS = 39.3
+OPS_PER_CELL = {"sw": 10, "swt": 10, "nwt": 9}   # algorithmic int ops / cell
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def host_cores():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit():
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def roofline(kstats, step_ms):
+    """Dominant DP kernel class: algorithmic ops per launch / launch time."""
+    if not kstats:
+        return None
+    top = max(kstats, key=lambda k: k["ms"])
+    mode = top["name"].split(".")[0]
+    ops = OPS_PER_CELL[mode] * top["cells"]
+    tops = ops / (top["ms"] * 1e-3) / 1e12 if top["ms"] > 0 else 0.0
+    peak = PEAK_I16_TOPS if top["name"].endswith("i16") else PEAK_I32_TOPS
+    return {"bound": "valu", "kernel": "k_dp<%s>" % top["name"],
+            "achieved": round(tops, 3), "peak": peak, "unit": "Tops/s",
+            "frac": round(tops / peak, 4), "traffic": None,
+            "ops_per_cell": OPS_PER_CELL[mode],
+            "launch_ms_avg": round(top["ms"] / top["launches"], 4),
+            "cells_per_launch": int(top["cells"] / top["launches"]),
+            "share_of_step": round(top["ms"] / max(step_ms, 1e-9), 3)}
+
+
+def cpu_baseline_xdrop(w, pairs, p, budget_s, threads):
+    from oracle import oracle as orc
+    rng = np.random.default_rng(1234)
+    # size the sample from a short probe so that the run takes ~budget_s
+    probe = pairs[rng.choice(len(pairs), min(2000, len(pairs)),
+                             replace=False)]
+    t = time.time()
+    orc.sxa_batch(w.codes, w.offsets, probe["r"], probe["c"], probe["seed"],
+                  p, nthreads=threads)
+    dt = max(time.time() - t, 1e-3)
+    n = int(min(len(pairs), max(len(probe), len(probe) * budget_s / dt)))
+    idx = np.sort(rng.choice(len(pairs), n, replace=False))
+    sample = pairs[idx]
+    t = time.time()
+    ref = orc.sxa_batch(w.codes, w.offsets, sample["r"], sample["c"],
+                        sample["seed"], p, nthreads=threads)
+    dt = time.time() - t
+    L = w.lengths
+    nw_cells = 0
+    for s in range(p.seed_cnt):
+        e = ref["ext"][:, s]
+        nw_cells += int(np.sum((e["end_h"] - e["beg_h"]).astype(np.int64) *
+                               (e["end_v"] - e["beg_v"]).astype(np.int64)))
+    x_cells = int(ref["ext"]["cells"].sum())
+    cells = nw_cells + x_cells
+    return idx, ref, {
+        "value": round(cells / dt / 1e9, 4), "unit": "GCUPS",
+        "pairs_per_s": round(n / dt, 1), "cores": threads, "kind": "port",
+        "sample": "%d of the %d pairs drawn uniformly (seed 1234), %.1f s, "
+                  "%d DP cells; oracle/pastis_oracle.c sxa path" %
+                  (n, len(pairs), dt, cells)}
+
+
+def cpu_baseline_sw(w, pairs, go, ge, budget_s, threads):
+    from oracle import oracle as orc
+    rng = np.random.default_rng(1234)
+    probe = pairs[rng.choice(len(pairs), min(500, len(pairs)), replace=False)]
+    t = time.time()
+    orc.sw_batch(w.codes, w.offsets, probe["r"], probe["c"], go, ge,
+                 nthreads=threads)
+    dt = max(time.time() - t, 1e-3)
+    n = int(min(len(pairs), max(500, len(probe) * budget_s / dt)))
+    idx = np.sort(rng.choice(len(pairs), n, replace=False))
+    sample = pairs[idx]
+    t = time.time()
+    ref = orc.sw_batch(w.codes, w.offsets, sample["r"], sample["c"], go, ge,
+                       nthreads=threads)
+    dt = time.time() - t
+    L = w.lengths
+    cells = int(np.sum(L[sample["r"]] * L[sample["c"]]))
+    return idx, ref, {
+        "value": round(cells / dt / 1e9, 4), "unit": "GCUPS",
+        "pairs_per_s": round(n / dt, 1), "cores": threads, "kind": "port",
+        "sample": "%d of the %d pairs drawn uniformly (seed 1234), %.1f s, "
+                  "%d cells; oracle/pastis_oracle.c local score path" %
+                  (n, len(pairs), dt, cells)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=["xdrop", "sw"], default="xdrop")
+    ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--gap-model", default="as_called",
+                    choices=["as_called", "as_documented"])
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU "
+              "path (pairs shard by index; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from pastis_amd import synth
+    from pastis_amd._lib import PAIR_DTYPE, RESULT_DTYPE, SCORE_DTYPE
+    from pastis_amd.aligner import Aligner, config_for_cli
+
+    t = time.time()
+    if args.workload == "xdrop":
+        w = synth.config2(scale=args.scale, threads=host_cores())
+        cfg = config_for_cli(-11, -2, 6, 49, 2, args.gap_model)
+    else:
+        w = synth.config1(scale=args.scale, threads=host_cores())
+        cfg = config_for_cli(-11, -2, 6, 0, 2, args.gap_model)
+    pairs = w.pairs
+    n = len(pairs)
+    log("workload %s: %d sequences, %d pairs, generated in %.1f s" %
+        (w.name, len(w.offsets) - 1, n, time.time() - t))
+
+    al = Aligner(cfg)
+    al.load_seqs(w.codes, w.offsets)
+    d_pairs = al.device_alloc(n * PAIR_DTYPE.itemsize)
+    out_dtype = RESULT_DTYPE if args.workload == "xdrop" else SCORE_DTYPE
+    d_out = al.device_alloc(n * out_dtype.itemsize)
+    al.h2d(d_pairs, pairs)
+
+    def step():
+        if args.workload == "xdrop":
+            al.align_dev(d_pairs, n, d_out)
+        else:
+            al.sw_scores_dev(d_pairs, n, d_out)
+
+    for i in range(args.warmup):
+        step()
+        al.synchronize()
+        log("warmup step %d done" % i)
+    al.reset_stats()
+    al.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step()
+    al.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    st = al.stats()
+    ks = al.kernel_stats()
+    log("timed %d steps in %.3f s" % (args.steps, elapsed))
+    for k in sorted(ks, key=lambda k: -k["ms"]):
+        log("  %-16s launches %5d  ms %9.2f  cells %.3e" %
+            (k["name"], k["launches"], k["ms"], k["cells"]))
+    cells = st["dp_cells"] + st["xdrop_cells"]
+    gcups = cells / elapsed / 1e9
+    ms_per_step = elapsed * 1e3 / args.steps
+    out = np.zeros(n, out_dtype)
+    al.d2h(out, d_out)
+
+    line = {
+        "metric": METRIC, "value": round(gcups, 3), "unit": "GCUPS",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "int16 (packed; int32 re-run on saturation)",
+        "data": "synthetic",
+        "config": {"workload": w.name, "sequences": len(w.offsets) - 1,
+                   "pairs": n, "gap_model": args.gap_model,
+                   "scale": args.scale,
+                   "xdrop": 49 if args.workload == "xdrop" else None,
+                   "seed_count": 2, "k": 6, "matrix": "BLOSUM62"},
+        "pairs_per_s": round(n * args.steps / elapsed, 1),
+        "cells_per_step": int(cells / args.steps),
+        "xdrop_cells_per_step": int(st["xdrop_cells"] / args.steps),
+        "passing_pairs": int(((out["flags"] & 1) != 0).sum())
+        if args.workload == "xdrop" else None,
+        "roofline": roofline(ks, ms_per_step * args.steps),
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        threads = host_cores()
+        if args.workload == "xdrop":
+            from oracle import oracle as orc
+            g = ({"gap_open": -2, "gap_ext": -11, "xdrop_gap": -11}
+                 if args.gap_model == "as_called" else
+                 {"gap_open": -11, "gap_ext": -2, "xdrop_gap": -2})
+            p = orc.params(g["gap_open"], g["gap_ext"], g["xdrop_gap"], 49, 6,
+                           2, 0.7, 30)
+            idx, ref, cb = cpu_baseline_xdrop(w, pairs, p, args.cpu_seconds,
+                                              threads)
+            ok = (np.array_equal(out["score"][idx], ref["score"]) and
+                  np.array_equal(out["score_aln"][idx], ref["score_aln"]) and
+                  np.array_equal(out["cols"][idx].astype(np.int64),
+                                 ref["cols"].astype(np.int64)))
+        else:
+            go, ge = (-2, -11) if args.gap_model == "as_called" else (-11, -2)
+            idx, ref, cb = cpu_baseline_sw(w, pairs, go, ge, args.cpu_seconds,
+                                           threads)
+            ok = (np.array_equal(out["score"][idx], ref[0]) and
+                  np.array_equal(out["r_end"][idx].astype(np.int32), ref[1]) and
+                  np.array_equal(out["c_end"][idx].astype(np.int32), ref[2]))
+        line["cpu_baseline"] = cb
+        line["parity_on_cpu_sample"] = "bit-exact" if ok else "MISMATCH"
+    al.device_free(d_pairs)
+    al.device_free(d_out)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

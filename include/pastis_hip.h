@@ -199,9 +199,27 @@ int pa_memcpy_h2d(struct pa_ctx *ctx, void *dst, const void *src, size_t bytes);
 int pa_memcpy_d2h(struct pa_ctx *ctx, void *dst, const void *src, size_t bytes);
 int pa_synchronize(struct pa_ctx *ctx);
 
-/* ---- counters -------------------------------------------------------- */
+/* Per kernel class: DP kernels are split by mode (sw = local score only,
+ * swt = local traced, nwt = global traced), rows per lane (r2, r4, r8) and
+ * arithmetic (i16 packed, i32); "rerun" classes are the int32 re-runs of
+ * saturated tasks.  cells are DP cells of the tasks the class ran. */
+typedef struct pa_kernel_stat {
+    char name[24];
+    uint64_t launches;
+    uint64_t cells;
+    double ms;              /* summed device time of the class's launches */
+} pa_kernel_stat;
+
+/* ---- counters --------------------------------------------------------- */
+
+/* Timers and counters of the calls made on a context, the device-side
+ * counterpart of the reference's TimePod timers (inc/TraceUtils.hpp:25-217:
+ * align|pre, align|cpu, align|post) and aln_pairs counters. */
 int pa_get_stats(struct pa_ctx *ctx, pa_stats *out);
 int pa_reset_stats(struct pa_ctx *ctx);
+/* Fills up to cap entries; *n receives the number of classes with launches. */
+int pa_get_kernel_stats(struct pa_ctx *ctx, pa_kernel_stat *out, int cap,
+                        int *n);
 
 #ifdef __cplusplus
 }

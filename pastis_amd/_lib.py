@@ -26,6 +26,7 @@ EXPORTS = [
     "pa_sw_scores", "pa_sw_scores_dev", "pa_xdrop_extents", "pa_kmer_prune",
     "pa_device_alloc", "pa_device_free", "pa_memcpy_h2d", "pa_memcpy_d2h",
     "pa_synchronize", "pa_get_stats", "pa_reset_stats",
+    "pa_get_kernel_stats",
 ]
 
 PAIR_DTYPE = np.dtype([("r", "<u4"), ("c", "<u4"), ("count", "<u2"),
@@ -59,6 +60,11 @@ class PaStats(C.Structure):
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class PaKernelStat(C.Structure):
+    _fields_ = [("name", C.c_char * 24), ("launches", C.c_uint64),
+                ("cells", C.c_uint64), ("ms", C.c_double)]
 
 
 assert PAIR_DTYPE.itemsize == 20 and RESULT_DTYPE.itemsize == 36
@@ -105,6 +111,8 @@ def lib():
     L.pa_synchronize.argtypes = [vp]
     L.pa_get_stats.argtypes = [vp, C.POINTER(PaStats)]
     L.pa_reset_stats.argtypes = [vp]
+    L.pa_get_kernel_stats.argtypes = [vp, C.POINTER(PaKernelStat), i32,
+                                      C.POINTER(C.c_int)]
     _lib = L
     return L
 

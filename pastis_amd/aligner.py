@@ -123,6 +123,16 @@ class Aligner:
               "pa_get_stats")
         return s.as_dict()
 
+    def kernel_stats(self):
+        arr = (_lib.PaKernelStat * 64)()
+        n = C.c_int()
+        check(self._ctx, lib().pa_get_kernel_stats(self._ctx, arr, 64,
+                                                   C.byref(n)),
+              "pa_get_kernel_stats")
+        return [dict(name=arr[i].name.decode(), launches=arr[i].launches,
+                     cells=arr[i].cells, ms=arr[i].ms)
+                for i in range(min(n.value, 64))]
+
     def reset_stats(self):
         check(self._ctx, lib().pa_reset_stats(self._ctx), "pa_reset_stats")
 

@@ -448,9 +448,8 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
                     if (TRACE) {
                         typedef typename TraceWord<R>::T W;
                         W w = TraceWord<R>::pack(codes);
-                        const size_t rec =
-                            ((size_t)p * (size_t)(tlen + 130) + (size_t)(t - 1)) *
-                                64u + (size_t)lane;
+                        const uint64_t rec =
+                            trace_rec(qlen, tlen, R, p, t, lane);
                         *(W *)(tr + rec * R) = w;
                     }
                     if (LOCAL) {
@@ -559,8 +558,8 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
 namespace pa {
 namespace {
 
-__device__ inline int trace_code(const uint8_t *tr, int R, int tlen, int i,
-                                 int j)
+__device__ inline int trace_code(const uint8_t *tr, int R, int qlen,
+                                 int tlen, int i, int j)
 {
     const int rows = 128 * R;
     const int i0 = i - 1;
@@ -568,8 +567,7 @@ __device__ inline int trace_code(const uint8_t *tr, int R, int tlen, int i,
     const int l = rr / (2 * R), q = rr % (2 * R);
     const int half = q / R, r = q % R;
     const int t = j + 2 * l + half;
-    const size_t rec =
-        ((size_t)p * (size_t)(tlen + 130) + (size_t)(t - 1)) * 64u + (size_t)l;
+    const uint64_t rec = trace_rec(qlen, tlen, R, p, t, l);
     const int nib = half * R + r;
     const uint8_t b = tr[rec * R + (nib >> 1)];
     return (nib & 1) ? (b >> 4) : (b & 15);
@@ -611,7 +609,7 @@ __global__ __launch_bounds__(256) void k_traceback(TbArgs a)
                 }
                 break;
             }
-            const int c = trace_code(tr, R, tlen, i, j);
+            const int c = trace_code(tr, R, (int)T.qlen, tlen, i, j);
             if (state == 0) {
                 const int src = c & 3;
                 if (src == 3)

@@ -21,7 +21,8 @@ hipError_t build_nw_tasks(const XdTask *xt, uint64_t nt, const XdOut *xo,
                           unsigned long long *xcells, hipStream_t s);
 hipError_t bin_tasks(const DpTask *tasks, uint32_t a, uint32_t b, int local,
                      int maxs, int open, int ext, uint32_t *lists,
-                     uint32_t cap, uint32_t *counts, hipStream_t s);
+                     uint32_t cap, uint32_t *counts,
+                     unsigned long long *bin_cells, hipStream_t s);
 hipError_t trace_sizes(const DpTask *tasks, uint64_t n, uint64_t *sizes,
                        hipStream_t s);
 hipError_t empty_out(const DpTask *tasks, uint32_t a, uint32_t b, int local,

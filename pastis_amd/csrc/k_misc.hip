@@ -118,12 +118,14 @@ __device__ inline bool global_i16_ok(uint32_t q, uint32_t t, int maxs,
 __global__ void k_bin(const DpTask *__restrict__ tasks, uint32_t a, uint32_t b,
                       int local, int maxs, int open, int ext,
                       uint32_t *__restrict__ lists, uint32_t cap,
-                      uint32_t *counts)
+                      uint32_t *counts, unsigned long long *bin_cells)
 {
     const uint32_t i = a + blockIdx.x * blockDim.x + threadIdx.x;
     int bin = -1;
+    uint64_t cells = 0;
     if (i < b) {
         const DpTask t = tasks[i];
+        cells = (uint64_t)t.qlen * t.tlen;
         if (t.qlen > 0 && t.tlen > 0) {
             const int R = rows_per_lane(t.qlen);
             const int rc = R == 2 ? 0 : (R == 4 ? 1 : 2);
@@ -148,6 +150,10 @@ __global__ void k_bin(const DpTask *__restrict__ tasks, uint32_t a, uint32_t b,
                 __popcll(m & ((1ull << lane) - 1ull));
             lists[(size_t)bb * cap + base + rank] = i;
         }
+        uint64_t c = bin == bb ? cells : 0;
+        for (int off = 32; off >= 1; off >>= 1)
+            c += __shfl_xor(c, off);
+        atomicAdd(&bin_cells[bb], lane == leader ? (unsigned long long)c : 0ull);
     }
 }
 
@@ -374,12 +380,14 @@ hipError_t build_nw_tasks(const XdTask *xt, uint64_t nt, const XdOut *xo,
 
 hipError_t bin_tasks(const DpTask *tasks, uint32_t a, uint32_t b, int local,
                      int maxs, int open, int ext, uint32_t *lists,
-                     uint32_t cap, uint32_t *counts, hipStream_t s)
+                     uint32_t cap, uint32_t *counts,
+                     unsigned long long *bin_cells, hipStream_t s)
 {
     if (b <= a)
         return hipSuccess;
     hipLaunchKernelGGL(k_bin, dim3(blocks(b - a, 256)), dim3(256), 0, s, tasks,
-                       a, b, local, maxs, open, ext, lists, cap, counts);
+                       a, b, local, maxs, open, ext, lists, cap, counts,
+                       bin_cells);
     return hipGetLastError();
 }
 

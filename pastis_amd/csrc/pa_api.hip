@@ -117,6 +117,11 @@ struct pa_ctx {
         sat, ctr, trace, bnd, xdt, xdo, xds, ovf, cub, cells, dmat, ext;
     size_t trace_cap = 0;
     std::vector<uint64_t> h_toffs;
+    Buf bincells;                 // u64 cells per (DP mode, bin)
+    struct KStat {
+        uint64_t launches = 0;
+        double ms = 0;
+    } kstat[3][kBins + 3];        // DP mode x (bins, int32 re-runs per R)
     std::vector<Timer> pending;
     std::vector<hipEvent_t> free_events;
     pa_stats stats{};
@@ -203,6 +208,14 @@ void resolve_timers(pa_ctx *c)
         float ms = 0.f;
         hipEventSynchronize(t.b);
         hipEventElapsedTime(&ms, t.a, t.b);
+        if (t.kind >= 100) {
+            auto &k = c->kstat[(t.kind - 100) / 16][(t.kind - 100) % 16];
+            k.launches++;
+            k.ms += ms;
+            c->free_events.push_back(t.a);
+            c->free_events.push_back(t.b);
+            continue;
+        }
         switch (t.kind) {
         case 0: c->stats.ms_sw += ms; break;
         case 1: c->stats.ms_xdrop += ms; break;
@@ -220,8 +233,6 @@ int ensure_common(pa_ctx *c, uint64_t nt)
 {
     int rc;
     if ((rc = grow(c, c->ctr, C_N * sizeof(uint32_t))))
-        return rc;
-    if ((rc = grow(c, c->cells, 4 * sizeof(uint64_t))))
         return rc;
     if ((rc = grow(c, c->tasks, nt * sizeof(DpTask))))
         return rc;
@@ -311,6 +322,7 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
     uint32_t *ctr = P<uint32_t>(c->ctr);
     uint32_t *lists = P<uint32_t>(c->lists);
     uint32_t *sat = P<uint32_t>(c->sat);
+    const int dmode = local ? (trace ? 1 : 0) : 2;
     for (auto &ch : chunks) {
         const uint32_t a = (uint32_t)ch.first, b = (uint32_t)ch.second;
         uint64_t base = 0;
@@ -319,7 +331,10 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
         HIPCHK(c, fill_u32(ctr, 0, C_XDQ, s));
         HIPCHK(c, bin_tasks(P<DpTask>(c->tasks), a, b, local ? 1 : 0, c->maxs,
                             c->cfg.gap_open, c->cfg.gap_ext, lists,
-                            (uint32_t)nt, ctr + C_BIN, s));
+                            (uint32_t)nt, ctr + C_BIN,
+                            (unsigned long long *)P<uint64_t>(c->bincells) +
+                                dmode * kBins,
+                            s));
         HIPCHK(c, empty_out(P<DpTask>(c->tasks), a, b, local ? 1 : 0,
                             c->cfg.gap_open, c->cfg.gap_ext,
                             P<DpOut>(c->dpout), s));
@@ -346,7 +361,11 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
                     args.queue = ctr + C_BINQ + bin;
                     args.sat_list = sat + (size_t)rcl * nt;
                     args.sat_count = ctr + C_SAT + rcl;
-                    HIPCHK(c, launch_dp(R, local, trace, w == 0, args, grid, s));
+                    {
+                        Scope k(c, 100 + dmode * 16 + bin, s);
+                        HIPCHK(c, launch_dp(R, local, trace, w == 0, args,
+                                            grid, s));
+                    }
                     c->stats.launches++;
                 }
                 if (local) {
@@ -354,7 +373,11 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
                     args.list = sat + (size_t)rcl * nt;
                     args.count = ctr + C_SAT + rcl;
                     args.queue = ctr + C_SATQ + rcl;
-                    HIPCHK(c, launch_dp(R, local, trace, false, args, grid, s));
+                    {
+                        Scope k(c, 100 + dmode * 16 + kBins + rcl, s);
+                        HIPCHK(c, launch_dp(R, local, trace, false, args,
+                                            grid, s));
+                    }
                     HIPCHK(c, mark_flags(args.list, args.count, b - a,
                                          P<uint8_t>(c->tflags), 1, s));
                     c->stats.launches += 2;
@@ -690,9 +713,18 @@ int pa_create(const pa_config *cfg, pa_ctx **out)
         pa_destroy(c);
         return PA_ERR_NOMEM;
     }
+    if (grow(c, c->bincells, 3 * kBins * sizeof(uint64_t)) ||
+        hipMemset(c->bincells.p, 0, 3 * kBins * sizeof(uint64_t)) !=
+            hipSuccess ||
+        grow(c, c->cells, 4 * sizeof(uint64_t)) ||
+        hipMemset(c->cells.p, 0, 4 * sizeof(uint64_t)) != hipSuccess) {
+        pa_destroy(c);
+        return PA_ERR_NOMEM;
+    }
     size_t fr = 0, tot = 0;
     hipMemGetInfo(&fr, &tot);
-    c->trace_cap = std::min<size_t>((size_t)8 << 30, fr / 4);
+    // the trace store may take up to 40 % of free HBM (96 GB at most)
+    c->trace_cap = std::min<size_t>((size_t)96 << 30, (size_t)(fr * 0.4));
     if (const char *e = std::getenv("PA_TRACE_BYTES"))
         c->trace_cap = std::max<size_t>(std::strtoull(e, nullptr, 10), 1 << 20);
     *out = c;
@@ -715,7 +747,7 @@ void pa_destroy(pa_ctx *c)
                    &c->results, &c->scores, &c->tasks, &c->dpout, &c->tflags,
                    &c->sizes, &c->toffs, &c->lists, &c->sat, &c->ctr,
                    &c->trace, &c->bnd, &c->xdt, &c->xdo, &c->xds, &c->ovf,
-                   &c->cub, &c->cells, &c->dmat, &c->ext})
+                   &c->cub, &c->cells, &c->dmat, &c->ext, &c->bincells})
         release(*b);
     for (auto e : c->free_events)
         hipEventDestroy(e);
@@ -998,8 +1030,50 @@ int pa_reset_stats(pa_ctx *c)
     HIPCHK(c, hipStreamSynchronize(c->st));
     resolve_timers(c);
     c->stats = pa_stats{};
+    for (auto &m : c->kstat)
+        for (auto &k : m)
+            k = pa_ctx::KStat{};
     if (c->cells.p)
         HIPCHK(c, hipMemset(c->cells.p, 0, 32));
+    HIPCHK(c, hipMemset(c->bincells.p, 0, 3 * kBins * sizeof(uint64_t)));
+    return PA_OK;
+}
+
+int pa_get_kernel_stats(pa_ctx *c, pa_kernel_stat *out, int cap, int *n)
+{
+    if (!c || !n || (cap > 0 && !out))
+        return PA_ERR_ARG;
+    hipSetDevice(c->device);
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    resolve_timers(c);
+    uint64_t cells[3 * kBins];
+    HIPCHK(c, hipMemcpy(cells, c->bincells.p, sizeof(cells),
+                        hipMemcpyDeviceToHost));
+    static const char *mname[3] = {"sw", "swt", "nwt"};
+    int k = 0;
+    for (int m = 0; m < 3; ++m)
+        for (int b = 0; b < kBins + 3; ++b) {
+            const auto &ks = c->kstat[m][b];
+            if (!ks.launches)
+                continue;
+            if (k < cap) {
+                pa_kernel_stat &o = out[k];
+                std::memset(&o, 0, sizeof(o));
+                if (b < kBins)
+                    std::snprintf(o.name, sizeof(o.name), "%s.r%d.%s",
+                                  mname[m], b / 2 == 0 ? 2 : (b / 2 == 1 ? 4 : 8),
+                                  b % 2 ? "i32" : "i16");
+                else
+                    std::snprintf(o.name, sizeof(o.name), "%s.r%d.rerun",
+                                  mname[m],
+                                  b - kBins == 0 ? 2 : (b - kBins == 1 ? 4 : 8));
+                o.launches = ks.launches;
+                o.ms = ks.ms;
+                o.cells = b < kBins ? cells[m * kBins + b] : 0;
+            }
+            ++k;
+        }
+    *n = k;
     return PA_OK;
 }
 

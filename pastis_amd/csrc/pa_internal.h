@@ -71,13 +71,37 @@ struct TbArgs {
     int local;
 };
 
-// Bytes of trace a task needs when traced with R rows per lane.
+// Trace layout of a task traced with R rows per lane: pass after pass; a
+// pass holds one record of R bytes (2R 4-bit codes) per (step, active lane),
+// step-major.  Full passes have 64 active lanes and tlen + 127 steps; the
+// last pass has nact lanes and tlen + 2*(nact-1) + 1 steps.
+__host__ __device__ inline uint32_t trace_last_lanes(uint32_t qlen, int R)
+{
+    const uint32_t rows = 128u * (uint32_t)R;
+    const uint32_t rem = qlen - ((qlen - 1) / rows) * rows;
+    return (rem + 2u * R - 1u) / (2u * R);
+}
+
 __host__ __device__ inline uint64_t trace_bytes(uint32_t qlen, uint32_t tlen,
                                                 int R)
 {
-    uint32_t rows = 128u * (uint32_t)R;
-    uint32_t passes = (qlen + rows - 1) / rows;
-    return (uint64_t)passes * (uint64_t)(tlen + 130u) * 64u * (uint64_t)R;
+    const uint32_t rows = 128u * (uint32_t)R;
+    const uint32_t full = (qlen - 1) / rows;
+    const uint32_t nl = trace_last_lanes(qlen, R);
+    return ((uint64_t)full * (tlen + 127u) * 64u +
+            (uint64_t)(tlen + 2u * (nl - 1u) + 1u) * nl) *
+           (uint64_t)R;
+}
+
+// Record index (in units of R bytes) of (pass p, step t >= 1, lane).
+__host__ __device__ inline uint64_t trace_rec(uint32_t qlen, uint32_t tlen,
+                                              int R, uint32_t p, uint32_t t,
+                                              uint32_t lane)
+{
+    const uint32_t rows = 128u * (uint32_t)R;
+    const uint32_t full = (qlen - 1) / rows;
+    const uint32_t nact = p < full ? 64u : trace_last_lanes(qlen, R);
+    return (uint64_t)p * (tlen + 127u) * 64u + (uint64_t)(t - 1u) * nact + lane;
 }
 
 // Rows per lane chosen for a query length (256, 512 or 1024 rows per pass).
