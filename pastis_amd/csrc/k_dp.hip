@@ -267,7 +267,25 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
     constexpr int NEG = TRACE ? O::kNegTraced : O::kNeg;
     HIP_DYNAMIC_SHARED(uint32_t, prof32)
     const uint8_t *prof = (const uint8_t *)prof32;
+    // biased score matrix (score or 4*score+2, plus 128) after the profile;
+    // columns >= 25 and row 31 (padding rows) hold 0, i.e. -128
+    uint32_t *mb32 = prof32 + kProfChars * ROWS / 4;
     const int lane = threadIdx.x;
+    for (int w = lane; w < 256; w += 64) {
+        uint32_t x = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int q = w / 8, c = (w % 8) * 4 + b;
+            int val = 0;
+            if (q < kAlpha && c < kAlpha) {
+                const int sc = a.mat[q * 32 + c];
+                val = (TRACE ? 4 * sc + 2 : sc) + 128;
+            }
+            x |= (uint32_t)val << (8 * b);
+        }
+        mb32[w] = x;
+    }
+    __syncthreads();
     const uint32_t count = *a.count;
     const int open = a.gap_open * (1 << SH);
     const int ext = a.gap_ext * (1 << SH);
@@ -305,22 +323,38 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
         for (int p = 0; p < npass; ++p) {
             const int i0 = p * ROWS;
             __syncthreads();
-            // query profile of rows [i0, i0 + ROWS)
-            for (int idx = lane; idx < kProfChars * ROWS / 4; idx += 64) {
-                const int c = idx / (ROWS / 4);
-                const int r4 = (idx % (ROWS / 4)) * 4;
-                uint32_t w = 0;
+            // query profile of rows [i0, i0 + ROWS): each lane writes its own
+            // 2R rows, four rows (one dword per residue code) at a time,
+            // gathering bytes of four matrix rows with v_perm
 #pragma unroll
-                for (int b = 0; b < 4; ++b) {
-                    const int row = i0 + r4 + b;
-                    int val = -128;
-                    if (c < kAlpha && row < qlen) {
-                        const int s = a.mat[(int)qs[row] * 32 + c];
-                        val = TRACE ? 4 * s + 2 : s;
-                    }
-                    w |= (uint32_t)(val + 128) << (8 * b);
+            for (int g = 0; g < R / 2; ++g) {
+                const int row0 = i0 + 2 * R * lane + 4 * g;
+                int qc[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    qc[k] = row0 + k < qlen ? (int)qs[row0 + k] : 31;
+                uint32_t mr[4][8];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const uint4 *src = (const uint4 *)(mb32 + qc[k] * 8);
+                    const uint4 x0 = src[0], x1 = src[1];
+                    mr[k][0] = x0.x; mr[k][1] = x0.y; mr[k][2] = x0.z;
+                    mr[k][3] = x0.w; mr[k][4] = x1.x; mr[k][5] = x1.y;
+                    mr[k][6] = x1.z; mr[k][7] = x1.w;
                 }
-                prof32[idx] = w;
+                uint32_t *dst = prof32 + (2 * R * lane + 4 * g) / 4;
+#pragma unroll
+                for (int c = 0; c < kProfChars; ++c) {
+                    const int wd = c >> 2, sh = c & 3;
+                    const uint32_t sel =
+                        (uint32_t)sh | ((uint32_t)(4 + sh) << 8) | 0x0c0c0000u;
+                    const uint32_t x01 =
+                        __builtin_amdgcn_perm(mr[1][wd], mr[0][wd], sel);
+                    const uint32_t x23 =
+                        __builtin_amdgcn_perm(mr[3][wd], mr[2][wd], sel);
+                    dst[c * (ROWS / 4)] =
+                        __builtin_amdgcn_perm(x23, x01, 0x05040100u);
+                }
             }
             if (p > 0)
                 __threadfence();
@@ -649,7 +683,7 @@ __global__ __launch_bounds__(256) void k_traceback(TbArgs a)
 template <int R, bool L, bool T, bool I>
 hipError_t launch_one(const DpArgs &a, int grid, hipStream_t s)
 {
-    const size_t lds = (size_t)kProfChars * 128 * R;
+    const size_t lds = (size_t)kProfChars * 128 * R + 1024;
     hipLaunchKernelGGL((k_dp<R, L, T, I>), dim3(grid), dim3(64), lds, s, a);
     return hipGetLastError();
 }

@@ -93,9 +93,8 @@ enum {
 };
 
 constexpr int kDpGrid = 2048;      // persistent DP workgroups (one wave each)
-constexpr int kXdBlocks = 512;     // X-drop workgroups of 256 lanes
-constexpr uint32_t kXdCap = 256;   // anti-diagonal slice per lane
-constexpr int kXdWideLanes = 4096; // lanes of the wide re-run
+constexpr int kXdRingBlocks = 4096;  // X-drop LDS-ring workgroups (one wave)
+constexpr int kXdWideLanes = 65536;  // lanes of the wide (global scratch) re-run
 
 }  // namespace
 
@@ -474,23 +473,20 @@ int run_xdrop(pa_ctx *c, const pa_pair *d_pairs, uint64_t n, hipStream_t s)
         return rc;
     if ((rc = grow(c, c->ovf, nt * sizeof(uint32_t))))
         return rc;
-    const size_t lanes = (size_t)kXdBlocks * 256;
-    if ((rc = grow(c, c->xds, lanes * 3 * kXdCap * sizeof(int32_t))))
-        return rc;
     uint32_t *ctr = P<uint32_t>(c->ctr);
     HIPCHK(c, fill_u32(ctr + C_XDQ, 0, C_N - C_XDQ, s));
     HIPCHK(c, fill_u32(ctr + C_XDN, (uint32_t)nt, 1, s));
     HIPCHK(c, build_xd_tasks(d_pairs, n, sc, P<uint64_t>(c->roffs),
                              P<uint64_t>(c->coffs), P<XdTask>(c->xdt), s));
     {
+        // LDS-ring kernel; bands wider than the ring go to the overflow list
         Scope t(c, 1, s);
-        const int blocks = (int)std::min<uint64_t>((nt + 255) / 256, kXdBlocks);
-        HIPCHK(c, launch_xdrop_k(P<XdTask>(c->xdt), nullptr, ctr + C_XDN,
-                                 ctr + C_XDQ, nullptr, P<int8_t>(c->dmat),
-                                 c->cfg.xdrop_gap, c->cfg.xdrop,
-                                 P<int32_t>(c->xds), kXdCap, P<XdOut>(c->xdo),
-                                 P<uint32_t>(c->ovf), ctr + C_XDOVF,
-                                 c->cfg.seed_len, blocks, s));
+        const int blocks = (int)std::min<uint64_t>((nt + 63) / 64, kXdRingBlocks);
+        HIPCHK(c, launch_xdrop_ring(P<XdTask>(c->xdt), ctr + C_XDN,
+                                    ctr + C_XDQ, P<int8_t>(c->dmat),
+                                    c->cfg.xdrop_gap, c->cfg.xdrop,
+                                    P<XdOut>(c->xdo), P<uint32_t>(c->ovf),
+                                    ctr + C_XDOVF, c->cfg.seed_len, blocks, s));
         c->stats.launches++;
     }
     uint32_t novf = 0;

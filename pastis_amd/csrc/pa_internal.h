@@ -138,5 +138,10 @@ hipError_t launch_xdrop_k(const XdTask *tasks, const uint32_t *list,
                           int xdrop, int32_t *scratch, uint32_t cap,
                           XdOut *out, uint32_t *ovf_list, uint32_t *ovf_count,
                           int seed_len, int grid, hipStream_t s);
+hipError_t launch_xdrop_ring(const XdTask *tasks, const uint32_t *count,
+                             uint32_t *queue, const int8_t *mat, int gap,
+                             int xdrop, XdOut *out, uint32_t *ovf_list,
+                             uint32_t *ovf_count, int seed_len, int grid,
+                             hipStream_t s);
 
 }  // namespace pa

@@ -35,6 +35,9 @@ struct uint2 {
     unsigned x, y;
 };
 inline uint2 make_uint2(unsigned a, unsigned b) { return {a, b}; }
+struct uint4 {
+    unsigned x, y, z, w;
+};
 
 typedef int hipError_t;
 constexpr hipError_t hipSuccess = 0;
