@@ -402,7 +402,11 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
             const int cap_step = tlen + 2 * cap_lane + cap_half;
             const bool last_pass = p == npass - 1;
 
-            for (int t = 1; t <= last_step; ++t) {
+            // trace record of (step 1, this lane); one step adds rstep records
+            uint64_t rec = TRACE ? trace_rec(qlen, tlen, R, p, 1, lane) : 0;
+            const uint32_t rstep = TRACE ? (p + 1 < npass ? 64u : (uint32_t)nact)
+                                         : 0u;
+            for (int t = 1; t <= last_step; ++t, rec += rstep) {
                 // column residues: lane 0 reads, the others take lane-1's
                 // residue of the previous step (two columns back for them)
                 // (no lane-dependent branches around the DPP moves: lane 0's
@@ -482,8 +486,6 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
                     if (TRACE) {
                         typedef typename TraceWord<R>::T W;
                         W w = TraceWord<R>::pack(codes);
-                        const uint64_t rec =
-                            trace_rec(qlen, tlen, R, p, t, lane);
                         *(W *)(tr + rec * R) = w;
                     }
                     if (LOCAL) {

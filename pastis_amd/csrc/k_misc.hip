@@ -89,7 +89,7 @@ __global__ void k_build_nw_tasks(const XdTask *__restrict__ xt, uint64_t nt,
         t.flags = 0;
         tasks[k] = t;
         c = (uint64_t)t.qlen * t.tlen;
-        xc = o.cells;
+        xc = (uint64_t)o.cells_left + o.cells_right;
     }
     for (int off = 32; off >= 1; off >>= 1) {
         c += __shfl_xor(c, off);

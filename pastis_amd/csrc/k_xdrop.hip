@@ -1,13 +1,23 @@
 // k_xdrop.hip -- gapped X-drop seed extension (SeqAn's GappedXDrop on a
 // simple seed, as called at src/align/SeqanXdropAligner.cpp:281-283).
 //
-// One extension task (one seed of one pair, both directions) per lane.  Each
-// lane keeps the three live anti-diagonals in its own slice of a global
-// scratch array (cached in L1/L2); the band update, the clipping at the
+// A work item is one direction (left or right) of one seed of one pair; the
+// two directions of a seed are independent and write disjoint fields of its
+// XdOut.  One work item per lane.  The band update, the clipping at the
 // segment ends and the "longest extension" end point follow the oracle's
-// or_xdrop_one step for step, so the extents and the evaluated-cell count are
-// bit-identical.  A band wider than the slice marks the task for a re-run
-// with a larger slice.
+// or_xdrop_one step for step, so extents and evaluated-cell counts are
+// bit-identical.
+//
+// Storage of the three live anti-diagonals: a ring per anti-diagonal indexed
+// by absolute column (slot = column mod ring size).  The algorithm only reads
+// columns inside the range written on that anti-diagonal ([off, maxCol]), so
+// the ring is exact while that range fits; a wider range reports overflow.
+//   - k_xdrop_lds: 32-slot rings in LDS ([3][32][64 lanes] int32);
+//   - k_xdrop_wide: rings in global scratch sized to the longest sequence
+//     (the overflow path).
+// Scheduling: a first pass runs with an anti-diagonal budget; items still
+// alive when it runs out are recomputed from scratch by a second pass, so
+// each pass keeps lanes of a wave on similar amounts of work.
 #include <limits.h>
 
 #include "pa_internal.h"
@@ -16,240 +26,52 @@ namespace pa {
 namespace {
 
 constexpr int32_t kUndef = INT_MIN / 4;
+constexpr int kRing = 32;
 
-struct OneDir {
+enum : uint16_t { kXdOvf = 1, kXdLong = 2 };
+
+struct Rings {
+    int32_t *base;
+    int ring_stride;   // elements between the three rings
+    int slot_stride;   // elements between consecutive slots
+    int mask;          // slots - 1
+    int slots;
+    __device__ inline int32_t &at(int r, int col) const
+    {
+        return base[r * ring_stride + (col & mask) * slot_stride];
+    }
+};
+
+struct Dir {
     int32_t score;
     int32_t col, row;
     bool upd;
-    bool ovf;
+    uint16_t status;   // 0 done, kXdOvf, kXdLong
 };
 
-__device__ OneDir xd_one(const uint8_t *qs, int qlen, const uint8_t *ds,
-                         int dlen, bool right, const int8_t *mat, int gap,
-                         int X, int32_t *buf, uint32_t cap, uint32_t *cells)
+__device__ Dir xd_dir(const uint8_t *qs, int qlen, const uint8_t *ds, int dlen,
+                      bool right, const int8_t *mat, int gap, int X,
+                      const Rings &A, int budget, uint32_t *cells)
 {
-    OneDir res = {0, 0, 0, false, false};
+    Dir res = {0, 0, 0, false, 0};
     const int cols = qlen + 1, rows = dlen + 1;
     if (rows == 1 || cols == 1)
         return res;
-    int32_t *a1 = buf, *a2 = buf + cap, *a3 = buf + 2 * cap;
-    int l1 = 0, l2 = 1, l3 = 2;
-    a2[0] = 0;
-    if (-gap > X) {
-        a3[0] = kUndef;
-        a3[1] = kUndef;
-    } else {
-        a3[0] = gap;
-        a3[1] = gap;
-    }
-    int minCol = 1, maxCol = 2;
-    int off1 = 0, off2 = 0, off3 = 0;
-    int d = 1;
-    int32_t best = 0;
-    uint32_t nc = 0;
-    while (minCol < maxCol) {
-        ++d;
-        int32_t *t = a1;
-        a1 = a2;
-        a2 = a3;
-        a3 = t;
-        l1 = l2;
-        l2 = l3;
-        off1 = off2;
-        off2 = off3;
-        off3 = minCol - 1;
-        l3 = maxCol + 1 - off3;
-        if ((uint32_t)l3 > cap) {
-            res.ovf = true;
-            return res;
-        }
-        a3[0] = kUndef;
-        a3[maxCol - off3] = kUndef;
-        if ((long)d * gap > (long)best - X) {
-            if (off3 == 0)
-                a3[0] = d * gap;
-            if (d - maxCol == 0)
-                a3[maxCol - off3] = d * gap;
-        }
-        int32_t adbest = d * gap;
-        const int32_t lim = best - X;
-        for (int col = minCol; col < maxCol; ++col) {
-            const int i3 = col - off3, i2 = col - off2, i1 = col - off1;
-            int qp, dp;
-            if (right) {
-                qp = col - 1;
-                dp = d - col - 1;
-            } else {
-                qp = cols - 1 - col;
-                dp = rows - 1 + col - d;
-            }
-            const int32_t l = a2[i2 - 1], u = a2[i2];
-            int32_t tmp = (l > u ? l : u) + gap;
-            const int32_t dg = a1[i1 - 1] + mat[(int)qs[qp] * 32 + ds[dp]];
-            tmp = dg > tmp ? dg : tmp;
-            if (tmp < lim) {
-                a3[i3] = kUndef;
-            } else {
-                a3[i3] = tmp;
-                adbest = tmp > adbest ? tmp : adbest;
-            }
-        }
-        nc += (uint32_t)(maxCol - minCol);
-        best = adbest > best ? adbest : best;
-        while (minCol - off3 < l3 && a3[minCol - off3] == kUndef &&
-               minCol - off2 - 1 < l2 && a2[minCol - off2 - 1] == kUndef)
-            ++minCol;
-        while (maxCol - off3 > 0 && a3[maxCol - off3 - 1] == kUndef &&
-               a2[maxCol - off2 - 1] == kUndef)
-            --maxCol;
-        ++maxCol;
-        if (minCol < d + 2 - rows)
-            minCol = d + 2 - rows;
-        if (maxCol > cols)
-            maxCol = cols;
-    }
-    *cells += nc;
-    int lcol = l3 + off3 - 2;
-    int lrow = d - lcol;
-    int32_t ls = a3[lcol - off3];
-    if (ls == kUndef) {
-        if (a2[l2 - 2] != kUndef) {
-            lcol = l2 + off2 - 2;
-            lrow = d - 1 - lcol;
-            ls = a2[lcol - off2];
-        } else if (l2 > 2 && a2[l2 - 3] != kUndef) {
-            lcol = l2 + off2 - 3;
-            lrow = d - 1 - lcol;
-            ls = a2[lcol - off2];
-        }
-    }
-    if (ls == kUndef) {
-        for (int i = 0; i < l1; ++i) {
-            if (a1[i] > ls) {
-                ls = a1[i];
-                lcol = i + off1;
-                lrow = d - 2 - lcol;
-            }
-        }
-    }
-    res.score = ls;
-    if (ls != kUndef) {
-        res.upd = true;
-        res.col = lcol;
-        res.row = lrow;
-    }
-    return res;
-}
-
-__global__ __launch_bounds__(256) void k_xdrop(
-    const XdTask *__restrict__ tasks, const uint32_t *__restrict__ list,
-    const uint32_t *count, uint32_t *queue, const uint8_t *__restrict__ codes,
-    const int8_t *__restrict__ gmat, int gap, int X, int32_t *scratch,
-    uint32_t cap, XdOut *out, uint32_t *ovf_list, uint32_t *ovf_count,
-    int seed_len)
-{
-    __shared__ int8_t mat[32 * 32];
-    for (int i = threadIdx.x; i < 32 * 32; i += blockDim.x)
-        mat[i] = gmat[i];
-    __syncthreads();
-    const uint32_t n = *count;
-    const int lane = threadIdx.x & 63;
-    int32_t *buf = scratch + (size_t)(blockIdx.x * blockDim.x + threadIdx.x) *
-                                 (size_t)(3 * cap);
-    for (;;) {
-        // all lanes join the atomic (lane 0 adds 64) and the loop body has
-        // no divergent continue, so the wave re-enters the loop together
-        uint32_t base = atomicAdd(queue, lane == 0 ? 64u : 0u);
-        base = __builtin_amdgcn_readfirstlane(base);
-        if (base >= n)
-            break;
-        const uint32_t k = base + (uint32_t)lane;
-        if (k < n) {
-        const uint32_t tid = list ? list[k] : k;
-        const XdTask T = tasks[tid];
-        const uint8_t *h = (const uint8_t *)(uintptr_t)T.h_off;
-        const uint8_t *v = (const uint8_t *)(uintptr_t)T.v_off;
-        int beg_h = (int)T.beg_h, beg_v = (int)T.beg_v;
-        int end_h = beg_h + seed_len, end_v = beg_v + seed_len;
-        uint32_t cells = 0;
-        XdOut o = {};
-        // left: query = V[0, beg_v), database = H[0, beg_h), reversed
-        OneDir L = xd_one(v, beg_v, h, beg_h, false, mat, gap, X, buf, cap,
-                          &cells);
-        OneDir Rr = {0, 0, 0, false, false};
-        if (!L.ovf)
-            Rr = xd_one(v + end_v, (int)T.nv - end_v, h + end_h,
-                        (int)T.nh - end_h, true, mat, gap, X, buf, cap,
-                        &cells);
-        if (L.ovf || Rr.ovf) {
-            o.flags = 1;
-            const uint32_t pos = atomicAdd(ovf_count, 1u);  // no wave op follows
-            ovf_list[pos] = tid;
-        } else {
-            if (L.upd) {
-                beg_h -= L.row;
-                beg_v -= L.col;
-            }
-            if (Rr.upd) {
-                end_h += Rr.row;
-                end_v += Rr.col;
-            }
-            o.beg_h = beg_h;
-            o.beg_v = beg_v;
-            o.end_h = end_h;
-            o.end_v = end_v;
-            o.score_left = L.score;
-            o.score_right = Rr.score;
-            o.cells = cells;
-        }
-        out[T.id] = o;
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// LDS-ring variant.  Each lane keeps the three live anti-diagonals in its own
-// column ring of kRing slots in LDS: the value of column c on an
-// anti-diagonal sits in slot c mod kRing of that anti-diagonal's ring.  The
-// algorithm only reads columns inside the range written on that anti-diagonal
-// ([off, maxCol]), so the ring is exact as long as that range is at most kRing
-// columns wide; a wider band marks the task for the global-scratch kernel.
-// Ring layout: [3 anti-diagonals][kRing slots][64 lanes] of int32.
-// ---------------------------------------------------------------------------
-constexpr int kRing = 32;
-
-struct Ring {
-    int32_t *base;  // LDS, this lane's column of the [3][kRing][64] array
-    __device__ inline int32_t &at(int r, int col) const
-    {
-        return base[(r * kRing + (col & (kRing - 1))) * 64];
-    }
-};
-
-__device__ OneDir xd_one_ring(const uint8_t *qs, int qlen, const uint8_t *ds,
-                              int dlen, bool right, const int8_t *mat, int gap,
-                              int X, const Ring &A, uint32_t *cells)
-{
-    OneDir res = {0, 0, 0, false, false};
-    const int cols = qlen + 1, rows = dlen + 1;
-    if (rows == 1 || cols == 1)
-        return res;
-    int r1 = 0, r2 = 1, r3 = 2;   // ring of anti-diagonals d-2, d-1, d
+    int r1 = 0, r2 = 1, r3 = 2;   // rings of anti-diagonals d-2, d-1, d
     int l1 = 0, l2 = 1, l3 = 2;
     A.at(r2, 0) = 0;
-    if (-gap > X) {
-        A.at(r3, 0) = kUndef;
-        A.at(r3, 1) = kUndef;
-    } else {
-        A.at(r3, 0) = gap;
-        A.at(r3, 1) = gap;
-    }
+    A.at(r3, 0) = -gap > X ? kUndef : gap;
+    A.at(r3, 1) = -gap > X ? kUndef : gap;
     int minCol = 1, maxCol = 2;
     int off1 = 0, off2 = 0, off3 = 0;
     int d = 1;
     int32_t best = 0;
     uint32_t nc = 0;
     while (minCol < maxCol) {
+        if (budget > 0 && d > budget) {
+            res.status = kXdLong;
+            return res;
+        }
         ++d;
         const int t = r1;
         r1 = r2;
@@ -261,10 +83,11 @@ __device__ OneDir xd_one_ring(const uint8_t *qs, int qlen, const uint8_t *ds,
         off2 = off3;
         off3 = minCol - 1;
         l3 = maxCol + 1 - off3;
-        if (l3 > kRing) {
-            res.ovf = true;
+        if (l3 > A.slots) {
+            res.status = kXdOvf;
             return res;
         }
+        // _initAntiDiag3: edge entries, first column / first row cells
         A.at(r3, off3) = kUndef;
         A.at(r3, maxCol) = kUndef;
         if ((long)d * gap > (long)best - X) {
@@ -311,6 +134,8 @@ __device__ OneDir xd_one_ring(const uint8_t *qs, int qlen, const uint8_t *ds,
             maxCol = cols;
     }
     *cells += nc;
+    // longest extension: both ends, end of query, end of database, else
+    // the best cell of the last live anti-diagonal
     int lcol = l3 + off3 - 2;
     int lrow = d - lcol;
     int32_t ls = A.at(r3, lcol);
@@ -344,90 +169,130 @@ __device__ OneDir xd_one_ring(const uint8_t *qs, int qlen, const uint8_t *ds,
     return res;
 }
 
-__global__ __launch_bounds__(64) void k_xdrop_ring(
-    const XdTask *__restrict__ tasks, const uint32_t *count, uint32_t *queue,
-    const int8_t *__restrict__ gmat, int gap, int X, XdOut *out,
-    uint32_t *ovf_list, uint32_t *ovf_count, int seed_len)
+// One work item: direction `dir` of extension task T.
+__device__ inline void run_item(const XdTask &T, int dir, const int8_t *mat,
+                                int gap, int X, int seed_len, const Rings &A,
+                                int budget, uint32_t k, XdOut *out,
+                                uint32_t *ovf_list, uint32_t *ovf_count,
+                                uint32_t *long_list, uint32_t *long_count)
+{
+    const uint8_t *h = (const uint8_t *)(uintptr_t)T.h_off;
+    const uint8_t *v = (const uint8_t *)(uintptr_t)T.v_off;
+    const int beg_h = (int)T.beg_h, beg_v = (int)T.beg_v;
+    const int end_h = beg_h + seed_len, end_v = beg_v + seed_len;
+    uint32_t cells = 0;
+    Dir r;
+    if (dir == 0)  // left: query = V[0, beg_v), database = H[0, beg_h)
+        r = xd_dir(v, beg_v, h, beg_h, false, mat, gap, X, A, budget, &cells);
+    else           // right: query = V[end_v, nv), database = H[end_h, nh)
+        r = xd_dir(v + end_v, (int)T.nv - end_v, h + end_h, (int)T.nh - end_h,
+                   true, mat, gap, X, A, budget, &cells);
+    XdOut &o = out[T.id];
+    if (r.status) {
+        uint32_t *lst = r.status == kXdOvf ? ovf_list : long_list;
+        uint32_t *cnt = r.status == kXdOvf ? ovf_count : long_count;
+        const uint32_t pos = atomicAdd(cnt, 1u);  // no wave op follows
+        lst[pos] = k;
+    }
+    if (dir == 0) {
+        o.beg_h = r.upd ? beg_h - r.row : beg_h;
+        o.beg_v = r.upd ? beg_v - r.col : beg_v;
+        o.score_left = r.score;
+        o.cells_left = cells;
+        o.flags_left = r.status;
+    } else {
+        o.end_h = r.upd ? end_h + r.row : end_h;
+        o.end_v = r.upd ? end_v + r.col : end_v;
+        o.score_right = r.score;
+        o.cells_right = cells;
+        o.flags_right = r.status;
+    }
+}
+
+struct XdArgs {
+    const XdTask *tasks;
+    const uint32_t *list;   // work items (k = 2 * task + dir), NULL = 0..count
+    const uint32_t *count;
+    uint32_t *queue;
+    const int8_t *mat;
+    int gap, X, seed_len, budget;
+    int32_t *scratch;       // wide kernel only
+    int cap;                // wide kernel: ring slots (power of two)
+    XdOut *out;
+    uint32_t *ovf_list, *ovf_count, *long_list, *long_count;
+};
+
+__global__ __launch_bounds__(64) void k_xdrop_lds(XdArgs a)
 {
     __shared__ int8_t mat[32 * 32];
     __shared__ int32_t ring[3 * kRing * 64];
     for (int i = threadIdx.x; i < 32 * 32; i += blockDim.x)
-        mat[i] = gmat[i];
+        mat[i] = a.mat[i];
     __syncthreads();
-    const uint32_t n = *count;
+    const uint32_t n = *a.count;
     const int lane = threadIdx.x;
-    const Ring A{ring + lane};
+    const Rings A{ring + lane, kRing * 64, 64, kRing - 1, kRing};
     for (;;) {
-        uint32_t base = atomicAdd(queue, lane == 0 ? 64u : 0u);
+        // all lanes join the atomic (lane 0 adds 64) and no divergent
+        // continue follows, so the wave re-enters the loop together
+        uint32_t base = atomicAdd(a.queue, lane == 0 ? 64u : 0u);
         base = __builtin_amdgcn_readfirstlane(base);
         if (base >= n)
             break;
-        const uint32_t k = base + (uint32_t)lane;
-        if (k < n) {
-            const XdTask T = tasks[k];
-            const uint8_t *h = (const uint8_t *)(uintptr_t)T.h_off;
-            const uint8_t *v = (const uint8_t *)(uintptr_t)T.v_off;
-            int beg_h = (int)T.beg_h, beg_v = (int)T.beg_v;
-            int end_h = beg_h + seed_len, end_v = beg_v + seed_len;
-            uint32_t cells = 0;
-            XdOut o = {};
-            OneDir L = xd_one_ring(v, beg_v, h, beg_h, false, mat, gap, X, A,
-                                   &cells);
-            OneDir Rr = {0, 0, 0, false, false};
-            if (!L.ovf)
-                Rr = xd_one_ring(v + end_v, (int)T.nv - end_v, h + end_h,
-                                 (int)T.nh - end_h, true, mat, gap, X, A,
-                                 &cells);
-            if (L.ovf || Rr.ovf) {
-                o.flags = 1;
-                const uint32_t pos = atomicAdd(ovf_count, 1u);
-                ovf_list[pos] = k;
-            } else {
-                if (L.upd) {
-                    beg_h -= L.row;
-                    beg_v -= L.col;
-                }
-                if (Rr.upd) {
-                    end_h += Rr.row;
-                    end_v += Rr.col;
-                }
-                o.beg_h = beg_h;
-                o.beg_v = beg_v;
-                o.end_h = end_h;
-                o.end_v = end_v;
-                o.score_left = L.score;
-                o.score_right = Rr.score;
-                o.cells = cells;
-            }
-            out[T.id] = o;
+        const uint32_t j = base + (uint32_t)lane;
+        if (j < n) {
+            const uint32_t k = a.list ? a.list[j] : j;
+            const XdTask T = a.tasks[k >> 1];
+            run_item(T, (int)(k & 1), mat, a.gap, a.X, a.seed_len, A, a.budget,
+                     k, a.out, a.ovf_list, a.ovf_count, a.long_list,
+                     a.long_count);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_xdrop_wide(XdArgs a)
+{
+    __shared__ int8_t mat[32 * 32];
+    for (int i = threadIdx.x; i < 32 * 32; i += blockDim.x)
+        mat[i] = a.mat[i];
+    __syncthreads();
+    const uint32_t n = *a.count;
+    const int lane = threadIdx.x & 63;
+    int32_t *buf = a.scratch + (size_t)(blockIdx.x * blockDim.x + threadIdx.x) *
+                                   (size_t)(3 * a.cap);
+    const Rings A{buf, a.cap, 1, a.cap - 1, a.cap};
+    for (;;) {
+        uint32_t base = atomicAdd(a.queue, lane == 0 ? 64u : 0u);
+        base = __builtin_amdgcn_readfirstlane(base);
+        if (base >= n)
+            break;
+        const uint32_t j = base + (uint32_t)lane;
+        if (j < n) {
+            const uint32_t k = a.list ? a.list[j] : j;
+            const XdTask T = a.tasks[k >> 1];
+            run_item(T, (int)(k & 1), mat, a.gap, a.X, a.seed_len, A, 0, k,
+                     a.out, a.ovf_list, a.ovf_count, a.long_list,
+                     a.long_count);
         }
     }
 }
 
 }  // namespace
 
-hipError_t launch_xdrop_k(const XdTask *tasks, const uint32_t *list,
-                          const uint32_t *count, uint32_t *queue,
-                          const uint8_t *codes, const int8_t *mat, int gap,
-                          int xdrop, int32_t *scratch, uint32_t cap,
-                          XdOut *out, uint32_t *ovf_list, uint32_t *ovf_count,
-                          int seed_len, int grid, hipStream_t s)
+hipError_t launch_xdrop(const XdTask *tasks, const uint32_t *list,
+                        const uint32_t *count, uint32_t *queue,
+                        const int8_t *mat, int gap, int xdrop, int seed_len,
+                        int budget, int32_t *scratch, int cap, XdOut *out,
+                        uint32_t *ovf_list, uint32_t *ovf_count,
+                        uint32_t *long_list, uint32_t *long_count, int grid,
+                        bool wide, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_xdrop, dim3(grid), dim3(256), 0, s, tasks, list,
-                       count, queue, codes, mat, gap, xdrop, scratch, cap, out,
-                       ovf_list, ovf_count, seed_len);
-    return hipGetLastError();
-}
-
-hipError_t launch_xdrop_ring(const XdTask *tasks, const uint32_t *count,
-                             uint32_t *queue, const int8_t *mat, int gap,
-                             int xdrop, XdOut *out, uint32_t *ovf_list,
-                             uint32_t *ovf_count, int seed_len, int grid,
-                             hipStream_t s)
-{
-    hipLaunchKernelGGL(k_xdrop_ring, dim3(grid), dim3(64), 0, s, tasks, count,
-                       queue, mat, gap, xdrop, out, ovf_list, ovf_count,
-                       seed_len);
+    XdArgs a{tasks, list, count, queue, mat, gap, xdrop, seed_len, budget,
+             scratch, cap, out, ovf_list, ovf_count, long_list, long_count};
+    if (wide)
+        hipLaunchKernelGGL(k_xdrop_wide, dim3(grid), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_xdrop_lds, dim3(grid), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 

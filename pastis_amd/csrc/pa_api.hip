@@ -89,11 +89,14 @@ enum {
     C_XDN = 21,         // X-drop task count
     C_XDDUMMY = 22,     // overflow sink of the wide re-run
     C_ERR = 23,         // validation errors
+    C_XDLONG = 24,      // X-drop items that outlived the first-pass budget
+    C_XDLONGQ = 25,     // their queue head
     C_N = 32
 };
 
 constexpr int kDpGrid = 2048;      // persistent DP workgroups (one wave each)
 constexpr int kXdRingBlocks = 4096;  // X-drop LDS-ring workgroups (one wave)
+constexpr int kXdBudget = 96;        // first-pass anti-diagonal budget
 constexpr int kXdWideLanes = 65536;  // lanes of the wide (global scratch) re-run
 
 }  // namespace
@@ -466,37 +469,48 @@ int run_xdrop(pa_ctx *c, const pa_pair *d_pairs, uint64_t n, hipStream_t s)
 {
     int rc;
     const int sc = c->cfg.seed_cnt;
-    const uint64_t nt = n * (uint64_t)sc;
+    const uint64_t nt = n * (uint64_t)sc;   // extension tasks
+    const uint64_t ni = 2 * nt;             // work items (two directions)
     if ((rc = grow(c, c->xdt, nt * sizeof(XdTask))))
         return rc;
     if ((rc = grow(c, c->xdo, 2 * n * sizeof(XdOut))))
         return rc;
-    if ((rc = grow(c, c->ovf, nt * sizeof(uint32_t))))
+    if ((rc = grow(c, c->ovf, 2 * ni * sizeof(uint32_t))))
         return rc;
     uint32_t *ctr = P<uint32_t>(c->ctr);
+    uint32_t *ovf = P<uint32_t>(c->ovf), *lng = ovf + ni;
     HIPCHK(c, fill_u32(ctr + C_XDQ, 0, C_N - C_XDQ, s));
-    HIPCHK(c, fill_u32(ctr + C_XDN, (uint32_t)nt, 1, s));
+    HIPCHK(c, fill_u32(ctr + C_XDN, (uint32_t)ni, 1, s));
     HIPCHK(c, build_xd_tasks(d_pairs, n, sc, P<uint64_t>(c->roffs),
                              P<uint64_t>(c->coffs), P<XdTask>(c->xdt), s));
     {
-        // LDS-ring kernel; bands wider than the ring go to the overflow list
         Scope t(c, 1, s);
-        const int blocks = (int)std::min<uint64_t>((nt + 63) / 64, kXdRingBlocks);
-        HIPCHK(c, launch_xdrop_ring(P<XdTask>(c->xdt), ctr + C_XDN,
-                                    ctr + C_XDQ, P<int8_t>(c->dmat),
-                                    c->cfg.xdrop_gap, c->cfg.xdrop,
-                                    P<XdOut>(c->xdo), P<uint32_t>(c->ovf),
-                                    ctr + C_XDOVF, c->cfg.seed_len, blocks, s));
-        c->stats.launches++;
+        const int grid = (int)std::min<uint64_t>((ni + 63) / 64, kXdRingBlocks);
+        // pass 1: anti-diagonal budget; pass 2: the items that outlived it
+        HIPCHK(c, launch_xdrop(P<XdTask>(c->xdt), nullptr, ctr + C_XDN,
+                               ctr + C_XDQ, P<int8_t>(c->dmat),
+                               c->cfg.xdrop_gap, c->cfg.xdrop,
+                               c->cfg.seed_len, kXdBudget, nullptr, 0,
+                               P<XdOut>(c->xdo), ovf, ctr + C_XDOVF, lng,
+                               ctr + C_XDLONG, grid, false, s));
+        HIPCHK(c, launch_xdrop(P<XdTask>(c->xdt), lng, ctr + C_XDLONG,
+                               ctr + C_XDLONGQ, P<int8_t>(c->dmat),
+                               c->cfg.xdrop_gap, c->cfg.xdrop,
+                               c->cfg.seed_len, 0, nullptr, 0,
+                               P<XdOut>(c->xdo), ovf, ctr + C_XDOVF, lng,
+                               ctr + C_XDDUMMY, grid, false, s));
+        c->stats.launches += 2;
     }
     uint32_t novf = 0;
     HIPCHK(c, hipMemcpyAsync(&novf, ctr + C_XDOVF, 4, hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipStreamSynchronize(s));
     if (novf) {
-        const uint32_t cap =
-            std::max(c->max_rlen, c->max_clen) + 4;
-        // every launched lane may take a task from the queue, so the scratch
-        // covers whole workgroups
+        // rings in global scratch holding a whole segment (power of two)
+        int cap = 64;
+        while (cap < (int)std::max(c->max_rlen, c->max_clen) + 4)
+            cap <<= 1;
+        // every launched lane may take an item, so the scratch covers whole
+        // workgroups
         const size_t blocks2 =
             (std::min<size_t>(novf, kXdWideLanes) + 255) / 256;
         const size_t lanes2 = blocks2 * 256;
@@ -505,13 +519,13 @@ int run_xdrop(pa_ctx *c, const pa_pair *d_pairs, uint64_t n, hipStream_t s)
             return rc;
         {
             Scope t(c, 1, s);
-            HIPCHK(c, launch_xdrop_k(P<XdTask>(c->xdt), P<uint32_t>(c->ovf),
-                                     ctr + C_XDOVF, ctr + C_XDOVFQ, nullptr,
-                                     P<int8_t>(c->dmat), c->cfg.xdrop_gap,
-                                     c->cfg.xdrop, P<int32_t>(wide), cap,
-                                     P<XdOut>(c->xdo), P<uint32_t>(c->ovf),
-                                     ctr + C_XDDUMMY, c->cfg.seed_len,
-                                     (int)blocks2, s));
+            HIPCHK(c, launch_xdrop(P<XdTask>(c->xdt), ovf, ctr + C_XDOVF,
+                                   ctr + C_XDOVFQ, P<int8_t>(c->dmat),
+                                   c->cfg.xdrop_gap, c->cfg.xdrop,
+                                   c->cfg.seed_len, 0, P<int32_t>(wide), cap,
+                                   P<XdOut>(c->xdo), lng, ctr + C_XDDUMMY,
+                                   lng, ctr + C_XDDUMMY, (int)blocks2, true,
+                                   s));
             c->stats.launches++;
         }
         HIPCHK(c, hipStreamSynchronize(s));

@@ -120,28 +120,27 @@ struct XdTask {
 };
 static_assert(sizeof(XdTask) == 40, "XdTask layout");
 
+// Seed extension result; the left direction writes beg_*, score_left,
+// cells_left, flags_left and the right direction the other half.
 struct XdOut {
     int32_t beg_h, beg_v, end_h, end_v;
     int32_t score_left, score_right;
-    uint32_t cells;
-    uint32_t flags;            // bit 0: band exceeded the scratch capacity
+    uint32_t cells_left, cells_right;
+    uint16_t flags_left, flags_right;  // 1 band too wide, 2 budget exhausted
+    uint32_t pad;
 };
-static_assert(sizeof(XdOut) == 32, "XdOut layout");
+static_assert(sizeof(XdOut) == 40, "XdOut layout");
 
 // Launchers (k_dp.hip, k_xdrop.hip, k_misc.hip).
 hipError_t launch_dp(int R, bool local, bool trace, bool i16,
                      const DpArgs &a, int grid, hipStream_t s);
 hipError_t launch_traceback(const TbArgs &a, int grid, hipStream_t s);
-hipError_t launch_xdrop_k(const XdTask *tasks, const uint32_t *list,
-                          const uint32_t *count, uint32_t *queue,
-                          const uint8_t *codes, const int8_t *mat, int gap,
-                          int xdrop, int32_t *scratch, uint32_t cap,
-                          XdOut *out, uint32_t *ovf_list, uint32_t *ovf_count,
-                          int seed_len, int grid, hipStream_t s);
-hipError_t launch_xdrop_ring(const XdTask *tasks, const uint32_t *count,
-                             uint32_t *queue, const int8_t *mat, int gap,
-                             int xdrop, XdOut *out, uint32_t *ovf_list,
-                             uint32_t *ovf_count, int seed_len, int grid,
-                             hipStream_t s);
+hipError_t launch_xdrop(const XdTask *tasks, const uint32_t *list,
+                        const uint32_t *count, uint32_t *queue,
+                        const int8_t *mat, int gap, int xdrop, int seed_len,
+                        int budget, int32_t *scratch, int cap, XdOut *out,
+                        uint32_t *ovf_list, uint32_t *ovf_count,
+                        uint32_t *long_list, uint32_t *long_count, int grid,
+                        bool wide, hipStream_t s);
 
 }  // namespace pa
