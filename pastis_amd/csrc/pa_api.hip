@@ -96,7 +96,7 @@ enum {
 
 constexpr int kDpGrid = 2048;      // persistent DP workgroups (one wave each)
 constexpr int kXdRingBlocks = 4096;  // X-drop LDS-ring workgroups (one wave)
-constexpr int kXdBudget = 96;        // first-pass anti-diagonal budget
+constexpr int kXdBudget = 0;         // first-pass anti-diagonal budget (0: none)
 constexpr int kXdWideLanes = 65536;  // lanes of the wide (global scratch) re-run
 
 }  // namespace
@@ -493,13 +493,16 @@ int run_xdrop(pa_ctx *c, const pa_pair *d_pairs, uint64_t n, hipStream_t s)
                                c->cfg.seed_len, kXdBudget, nullptr, 0,
                                P<XdOut>(c->xdo), ovf, ctr + C_XDOVF, lng,
                                ctr + C_XDLONG, grid, false, s));
-        HIPCHK(c, launch_xdrop(P<XdTask>(c->xdt), lng, ctr + C_XDLONG,
-                               ctr + C_XDLONGQ, P<int8_t>(c->dmat),
-                               c->cfg.xdrop_gap, c->cfg.xdrop,
-                               c->cfg.seed_len, 0, nullptr, 0,
-                               P<XdOut>(c->xdo), ovf, ctr + C_XDOVF, lng,
-                               ctr + C_XDDUMMY, grid, false, s));
-        c->stats.launches += 2;
+        if (kXdBudget > 0) {
+            HIPCHK(c, launch_xdrop(P<XdTask>(c->xdt), lng, ctr + C_XDLONG,
+                                   ctr + C_XDLONGQ, P<int8_t>(c->dmat),
+                                   c->cfg.xdrop_gap, c->cfg.xdrop,
+                                   c->cfg.seed_len, 0, nullptr, 0,
+                                   P<XdOut>(c->xdo), ovf, ctr + C_XDOVF, lng,
+                                   ctr + C_XDDUMMY, grid, false, s));
+            c->stats.launches++;
+        }
+        c->stats.launches++;
     }
     uint32_t novf = 0;
     HIPCHK(c, hipMemcpyAsync(&novf, ctr + C_XDOVF, 4, hipMemcpyDeviceToHost, s));
