@@ -192,6 +192,62 @@ int pa_xdrop_extents(struct pa_ctx *ctx, const pa_pair *pairs, uint64_t n,
 int pa_kmer_prune(const pa_pair *pairs, uint64_t n, int ckthr, float mosthr,
                   uint8_t *keep, uint64_t *kept);
 
+/* ---- learned prefilter ------------------------------------------------
+ * Stands beside the k-mer prefilter of src/DistPWRunner.cpp:48-54 (which
+ * prunes on count <= ckthr || score <= mosthr): an MLP 16 -> 64 -> 64 -> 1
+ * over per-pair features predicts whether the exact alignment will pass the
+ * coverage/identity predicate (SeqanXdropAligner.cpp:344-351,
+ * SeqanFullAligner.cpp:253-260).  It has no counterpart in the reference.
+ * Weights and activations are bf16, accumulation fp32.  Parameters are a flat
+ * fp32 array: W1[16][64] b1[64] W2[64][64] b2[64] w3[64] b3 (in-major W).
+ * Features are standardized as (f - mean) * istd before the first layer. */
+#define PA_FILTER_NF 16
+#define PA_FILTER_NPARAM 5313
+
+typedef struct pa_train_config {
+    uint32_t epochs;
+    uint32_t batch;         /* samples per Adam step, 1..65536 */
+    uint64_t seed;          /* shuffle seed of every epoch; 0 keeps order */
+    float lr, beta1, beta2, eps;
+    float pos_weight;       /* weight of positive samples in the BCE */
+    int standardize;        /* 1: set mean/istd from these features first */
+} pa_train_config;
+
+/* Features of pairs over the loaded sequence sets (n x 16 fp32). */
+int pa_filter_features(struct pa_ctx *ctx, const pa_pair *pairs, uint64_t n,
+                       float *feats);
+int pa_filter_features_dev(struct pa_ctx *ctx, const pa_pair *d_pairs,
+                           uint64_t n, float *d_feats, void *stream);
+/* Installs parameters (PA_FILTER_NPARAM floats) and the standardization
+ * (16 + 16 floats; NULL keeps the current one, initially mean 0, istd 1);
+ * resets the Adam state. */
+int pa_filter_set_model(struct pa_ctx *ctx, const float *params,
+                        const float *mean, const float *istd);
+int pa_filter_get_model(struct pa_ctx *ctx, float *params, float *mean,
+                        float *istd);
+/* Mean weighted BCE over the n samples (one batch, in order) and its
+ * gradient (PA_FILTER_NPARAM floats) at the current parameters; n <= 65536. */
+int pa_filter_grad(struct pa_ctx *ctx, const float *feats,
+                   const uint8_t *labels, uint64_t n, float pos_weight,
+                   float *grad, float *loss);
+/* Adam training on the GPU; epoch_loss (may be NULL) receives the mean loss
+ * of every epoch. */
+int pa_filter_train(struct pa_ctx *ctx, const float *feats,
+                    const uint8_t *labels, uint64_t n,
+                    const pa_train_config *tc, float *epoch_loss);
+/* Logits of precomputed features. */
+int pa_filter_logits(struct pa_ctx *ctx, const float *feats, uint64_t n,
+                     float *logits);
+/* Fused features + MLP on device pairs. */
+int pa_filter_score_dev(struct pa_ctx *ctx, const pa_pair *d_pairs,
+                        uint64_t n, float *d_logits, void *stream);
+/* Keeps the pairs whose logit is >= tau (kept order is unspecified;
+ * d_kept_index may be NULL); synchronizes to return *n_kept. */
+int pa_filter_select_dev(struct pa_ctx *ctx, const pa_pair *d_pairs,
+                         uint64_t n, float tau, pa_pair *d_kept,
+                         uint32_t *d_kept_index, uint64_t *n_kept,
+                         void *stream);
+
 /* ---- device memory helpers (for the _dev entry points) --------------- */
 int pa_device_alloc(struct pa_ctx *ctx, size_t bytes, void **ptr);
 int pa_device_free(struct pa_ctx *ctx, void *ptr);

@@ -26,8 +26,14 @@ EXPORTS = [
     "pa_sw_scores", "pa_sw_scores_dev", "pa_xdrop_extents", "pa_kmer_prune",
     "pa_device_alloc", "pa_device_free", "pa_memcpy_h2d", "pa_memcpy_d2h",
     "pa_synchronize", "pa_get_stats", "pa_reset_stats",
-    "pa_get_kernel_stats",
+    "pa_get_kernel_stats", "pa_filter_features", "pa_filter_features_dev",
+    "pa_filter_set_model", "pa_filter_get_model", "pa_filter_grad",
+    "pa_filter_train", "pa_filter_logits", "pa_filter_score_dev",
+    "pa_filter_select_dev",
 ]
+
+PA_FILTER_NF = 16
+PA_FILTER_NPARAM = 5313
 
 PAIR_DTYPE = np.dtype([("r", "<u4"), ("c", "<u4"), ("count", "<u2"),
                        ("kscore", "<i2"), ("seed", "<u2", (4,))])
@@ -65,6 +71,13 @@ class PaStats(C.Structure):
 class PaKernelStat(C.Structure):
     _fields_ = [("name", C.c_char * 24), ("launches", C.c_uint64),
                 ("cells", C.c_uint64), ("ms", C.c_double)]
+
+
+class PaTrainConfig(C.Structure):
+    _fields_ = [("epochs", C.c_uint32), ("batch", C.c_uint32),
+                ("seed", C.c_uint64), ("lr", C.c_float), ("beta1", C.c_float),
+                ("beta2", C.c_float), ("eps", C.c_float),
+                ("pos_weight", C.c_float), ("standardize", C.c_int32)]
 
 
 assert PAIR_DTYPE.itemsize == 20 and RESULT_DTYPE.itemsize == 36
@@ -113,6 +126,18 @@ def lib():
     L.pa_reset_stats.argtypes = [vp]
     L.pa_get_kernel_stats.argtypes = [vp, C.POINTER(PaKernelStat), i32,
                                       C.POINTER(C.c_int)]
+    fp = C.POINTER(C.c_float)
+    L.pa_filter_features.argtypes = [vp, vp, u64, vp]
+    L.pa_filter_features_dev.argtypes = [vp, vp, u64, vp, vp]
+    L.pa_filter_set_model.argtypes = [vp, vp, vp, vp]
+    L.pa_filter_get_model.argtypes = [vp, vp, vp, vp]
+    L.pa_filter_grad.argtypes = [vp, vp, vp, u64, C.c_float, vp, fp]
+    L.pa_filter_train.argtypes = [vp, vp, vp, u64, C.POINTER(PaTrainConfig),
+                                  vp]
+    L.pa_filter_logits.argtypes = [vp, vp, u64, vp]
+    L.pa_filter_score_dev.argtypes = [vp, vp, u64, vp, vp]
+    L.pa_filter_select_dev.argtypes = [vp, vp, u64, C.c_float, vp, vp,
+                                       C.POINTER(C.c_uint64), vp]
     _lib = L
     return L
 

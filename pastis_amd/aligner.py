@@ -136,6 +136,79 @@ class Aligner:
     def reset_stats(self):
         check(self._ctx, lib().pa_reset_stats(self._ctx), "pa_reset_stats")
 
+    # ---- learned prefilter ------------------------------------------------
+    def filter_features(self, pairs):
+        pairs = np.ascontiguousarray(pairs, PAIR_DTYPE)
+        out = np.zeros((len(pairs), _lib.PA_FILTER_NF), np.float32)
+        check(self._ctx, lib().pa_filter_features(self._ctx, _ptr(pairs),
+                                                  len(pairs), _ptr(out)),
+              "pa_filter_features")
+        return out
+
+    def filter_set_model(self, params, mean=None, istd=None):
+        params = np.ascontiguousarray(params, np.float32)
+        assert params.shape == (_lib.PA_FILTER_NPARAM,)
+        mean = None if mean is None else np.ascontiguousarray(mean, np.float32)
+        istd = None if istd is None else np.ascontiguousarray(istd, np.float32)
+        check(self._ctx, lib().pa_filter_set_model(self._ctx, _ptr(params),
+                                                   _ptr(mean), _ptr(istd)),
+              "pa_filter_set_model")
+
+    def filter_get_model(self):
+        p = np.zeros(_lib.PA_FILTER_NPARAM, np.float32)
+        m = np.zeros(_lib.PA_FILTER_NF, np.float32)
+        s = np.zeros(_lib.PA_FILTER_NF, np.float32)
+        check(self._ctx, lib().pa_filter_get_model(self._ctx, _ptr(p), _ptr(m),
+                                                   _ptr(s)),
+              "pa_filter_get_model")
+        return p, m, s
+
+    def filter_grad(self, feats, labels, pos_weight=1.0):
+        feats = np.ascontiguousarray(feats, np.float32)
+        labels = np.ascontiguousarray(labels, np.uint8)
+        g = np.zeros(_lib.PA_FILTER_NPARAM, np.float32)
+        loss = C.c_float()
+        check(self._ctx, lib().pa_filter_grad(self._ctx, _ptr(feats),
+                                              _ptr(labels), len(labels),
+                                              C.c_float(pos_weight), _ptr(g),
+                                              C.byref(loss)),
+              "pa_filter_grad")
+        return float(loss.value), g
+
+    def filter_train(self, feats, labels, epochs=1, batch=4096, seed=1,
+                     lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8,
+                     pos_weight=1.0, standardize=True):
+        feats = np.ascontiguousarray(feats, np.float32)
+        labels = np.ascontiguousarray(labels, np.uint8)
+        tc = _lib.PaTrainConfig(epochs, batch, seed, lr, beta1, beta2, eps,
+                                pos_weight, int(bool(standardize)))
+        losses = np.zeros(epochs, np.float32)
+        check(self._ctx, lib().pa_filter_train(self._ctx, _ptr(feats),
+                                               _ptr(labels), len(labels),
+                                               C.byref(tc), _ptr(losses)),
+              "pa_filter_train")
+        return losses
+
+    def filter_logits(self, feats):
+        feats = np.ascontiguousarray(feats, np.float32)
+        out = np.zeros(len(feats), np.float32)
+        check(self._ctx, lib().pa_filter_logits(self._ctx, _ptr(feats),
+                                                len(feats), _ptr(out)),
+              "pa_filter_logits")
+        return out
+
+    def filter_score_dev(self, d_pairs, n, d_logits):
+        check(self._ctx, lib().pa_filter_score_dev(self._ctx, d_pairs, n,
+                                                   d_logits, None),
+              "pa_filter_score_dev")
+
+    def filter_select_dev(self, d_pairs, n, tau, d_kept, d_kept_index=None):
+        kept = C.c_uint64()
+        check(self._ctx, lib().pa_filter_select_dev(
+            self._ctx, d_pairs, n, C.c_float(tau), d_kept, d_kept_index,
+            C.byref(kept), None), "pa_filter_select_dev")
+        return kept.value
+
 
 def kmer_prune(pairs, ckthr, mosthr):
     """DistPWRunner.cpp:48-54 k-mer prefilter: boolean keep mask."""

@@ -18,21 +18,13 @@
 #include <string>
 #include <vector>
 
+#include "k_filter.h"
 #include "k_misc.h"
+#include "pa_ctx.h"
 
 using namespace pa;
 
 namespace {
-
-struct Buf {
-    void *p = nullptr;
-    size_t cap = 0;
-};
-
-struct Timer {
-    int kind;  // 0 sw, 1 xdrop, 2 nw, 3 trace, 4 total
-    hipEvent_t a, b;
-};
 
 const char *kStatus[] = {"ok",
                          "invalid argument",
@@ -101,78 +93,9 @@ constexpr int kXdWideLanes = 65536;  // lanes of the wide (global scratch) re-ru
 
 }  // namespace
 
-struct pa_ctx {
-    pa_config cfg{};
-    int8_t mat[32 * 32];
-    int maxs = 0;
-    int device = 0;
-    hipStream_t st = nullptr;
-    std::string err;
-    // sequence sets
-    Buf rcodes, ccodes, roffs, coffs;
-    bool shared_cols = false;
-    std::vector<uint32_t> rlen, clen;
-    uint32_t max_rlen = 0, max_clen = 0;
-    bool have_rows = false, have_cols = false;
-    // workspace
-    Buf pairs, results, scores, tasks, dpout, tflags, sizes, toffs, lists,
-        sat, ctr, trace, bnd, xdt, xdo, xds, ovf, cub, cells, dmat, ext;
-    size_t trace_cap = 0;
-    std::vector<uint64_t> h_toffs;
-    Buf bincells;                 // u64 cells per (DP mode, bin)
-    struct KStat {
-        uint64_t launches = 0;
-        double ms = 0;
-    } kstat[3][kBins + 3];        // DP mode x (bins, int32 re-runs per R)
-    std::vector<Timer> pending;
-    std::vector<hipEvent_t> free_events;
-    pa_stats stats{};
-};
-
 namespace {
 
-int fail(pa_ctx *c, int code, const std::string &msg)
-{
-    if (c)
-        c->err = msg;
-    return code;
-}
 
-#define HIPCHK(ctx, x)                                                       \
-    do {                                                                     \
-        hipError_t e_ = (x);                                                 \
-        if (e_ != hipSuccess)                                                \
-            return fail((ctx), PA_ERR_HIP,                                   \
-                        std::string(#x) + ": " + hipGetErrorString(e_));     \
-    } while (0)
-
-int grow(pa_ctx *c, Buf &b, size_t bytes)
-{
-    if (b.cap >= bytes && b.p)
-        return PA_OK;
-    if (b.p)
-        hipFree(b.p);
-    b.p = nullptr;
-    b.cap = 0;
-    size_t want = bytes + bytes / 4 + 256;
-    if (hipMalloc(&b.p, want) != hipSuccess) {
-        b.p = nullptr;
-        return fail(c, PA_ERR_NOMEM,
-                    "device allocation of " + std::to_string(want) + " bytes");
-    }
-    b.cap = want;
-    return PA_OK;
-}
-
-void release(Buf &b)
-{
-    if (b.p)
-        hipFree(b.p);
-    b.p = nullptr;
-    b.cap = 0;
-}
-
-template <class T> T *P(Buf &b) { return (T *)b.p; }
 
 hipEvent_t get_event(pa_ctx *c)
 {
@@ -616,6 +539,10 @@ int upload_set(pa_ctx *c, bool rows, const uint8_t *res, const uint64_t *offs,
                              hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipMemcpyAsync(ob.p, abs.data(), abs.size() * 8,
                              hipMemcpyHostToDevice, c->st));
+    Buf &sb = rows ? c->rsk : c->csk;
+    if ((rc = grow(c, sb, std::max<uint64_t>(nseq, 1) * sizeof(SeqSketch))))
+        return rc;
+    HIPCHK(c, seq_sketch(P<uint64_t>(ob), nseq, P<SeqSketch>(sb), c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     if (rows) {
         c->rlen.swap(lens);
@@ -755,11 +682,15 @@ void pa_destroy(pa_ctx *c)
     if (c->shared_cols) {
         c->ccodes = Buf();
         c->coffs = Buf();
+        c->csk = Buf();
     }
+    release(c->csk);
+    filter_free(c->flt);
+    c->flt = nullptr;
     for (Buf *b : {&c->rcodes, &c->ccodes, &c->roffs, &c->coffs, &c->pairs,
                    &c->results, &c->scores, &c->tasks, &c->dpout, &c->tflags,
                    &c->sizes, &c->toffs, &c->lists, &c->sat, &c->ctr,
-                   &c->trace, &c->bnd, &c->xdt, &c->xdo, &c->xds, &c->ovf,
+                   &c->trace, &c->bnd, &c->xdt, &c->xdo, &c->ovf, &c->rsk,
                    &c->cub, &c->cells, &c->dmat, &c->ext, &c->bincells})
         release(*b);
     for (auto e : c->free_events)
@@ -784,6 +715,7 @@ int pa_load_seqs(pa_ctx *c, int side, const uint8_t *residues,
             // the column set aliased the row buffers; detach it first
             c->ccodes = Buf();
             c->coffs = Buf();
+            c->csk = Buf();
             c->have_cols = false;
             c->shared_cols = false;
         }
@@ -794,6 +726,7 @@ int pa_load_seqs(pa_ctx *c, int side, const uint8_t *residues,
         if (c->shared_cols) {
             c->ccodes = Buf();
             c->coffs = Buf();
+            c->csk = Buf();
             c->shared_cols = false;
         }
         if ((rc = upload_set(c, false, residues, offsets, nseq, encoding)))
@@ -803,9 +736,11 @@ int pa_load_seqs(pa_ctx *c, int side, const uint8_t *residues,
         if (!c->shared_cols) {
             release(c->ccodes);
             release(c->coffs);
+            release(c->csk);
         }
         c->ccodes = c->rcodes;
         c->coffs = c->roffs;
+        c->csk = c->rsk;
         c->clen = c->rlen;
         c->max_clen = c->max_rlen;
         c->have_cols = true;

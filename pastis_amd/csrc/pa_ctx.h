@@ -1,0 +1,104 @@
+// pa_ctx.h -- the context behind the C ABI's opaque pa_ctx handle, shared by
+// the host-side sources of libpastis_hip.so.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "k_misc.h"
+
+namespace pa {
+
+struct Buf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+struct Timer {
+    int kind;  // 0 sw, 1 xdrop, 2 nw, 3 trace, 4 total
+    hipEvent_t a, b;
+};
+
+struct FilterState;   // learned prefilter (pa_filter.hip)
+void filter_free(FilterState *f);
+
+}  // namespace pa
+
+struct pa_ctx {
+    using Buf = pa::Buf;
+    using Timer = pa::Timer;
+    pa_config cfg{};
+    int8_t mat[32 * 32];
+    int maxs = 0;
+    int device = 0;
+    hipStream_t st = nullptr;
+    std::string err;
+    // sequence sets
+    Buf rcodes, ccodes, roffs, coffs;
+    Buf rsk, csk;                 // per-sequence filter sketches
+    bool shared_cols = false;
+    std::vector<uint32_t> rlen, clen;
+    uint32_t max_rlen = 0, max_clen = 0;
+    bool have_rows = false, have_cols = false;
+    // workspace
+    Buf pairs, results, scores, tasks, dpout, tflags, sizes, toffs, lists,
+        sat, ctr, trace, bnd, xdt, xdo, ovf, cub, cells, dmat, ext;
+    size_t trace_cap = 0;
+    std::vector<uint64_t> h_toffs;
+    Buf bincells;                 // u64 cells per (DP mode, bin)
+    struct KStat {
+        uint64_t launches = 0;
+        double ms = 0;
+    } kstat[3][pa::kBins + 3];        // DP mode x (bins, int32 re-runs per R)
+    std::vector<Timer> pending;
+    std::vector<hipEvent_t> free_events;
+    pa_stats stats{};
+    pa::FilterState *flt = nullptr;
+};
+
+namespace pa {
+
+inline int fail(pa_ctx *c, int code, const std::string &msg)
+{
+    if (c)
+        c->err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, x)                                                       \
+    do {                                                                     \
+        hipError_t e_ = (x);                                                 \
+        if (e_ != hipSuccess)                                                \
+            return fail((ctx), PA_ERR_HIP,                                   \
+                        std::string(#x) + ": " + hipGetErrorString(e_));     \
+    } while (0)
+
+inline int grow(pa_ctx *c, Buf &b, size_t bytes)
+{
+    if (b.cap >= bytes && b.p)
+        return PA_OK;
+    if (b.p)
+        hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+    size_t want = bytes + bytes / 4 + 256;
+    if (hipMalloc(&b.p, want) != hipSuccess) {
+        b.p = nullptr;
+        return fail(c, PA_ERR_NOMEM,
+                    "device allocation of " + std::to_string(want) + " bytes");
+    }
+    b.cap = want;
+    return PA_OK;
+}
+
+inline void release(Buf &b)
+{
+    if (b.p)
+        hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+}
+
+template <class T> inline T *P(Buf &b) { return (T *)b.p; }
+
+}  // namespace pa

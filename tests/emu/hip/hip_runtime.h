@@ -11,6 +11,7 @@
 #pragma once
 
 #include <atomic>
+#include <cmath>
 #include <barrier>
 #include <cstdint>
 #include <cstdlib>
@@ -24,6 +25,7 @@
 #define __host__
 #define __launch_bounds__(...)
 #define __shared__ static
+#define __constant__ static const
 #define __forceinline__ inline
 #define HIP_DYNAMIC_SHARED(type, var) type *var = (type *)emu::dyn_shared();
 
@@ -115,6 +117,25 @@ template <class T> T __shfl_xor(T v, int m)
 inline uint64_t __ballot(bool p) { return emu::wave_ballot(p); }
 inline int __ffsll(unsigned long long x) { return __builtin_ffsll((long long)x); }
 inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
+inline float __uint_as_float(unsigned x)
+{
+    float f;
+    std::memcpy(&f, &x, 4);
+    return f;
+}
+inline unsigned __float_as_uint(float f)
+{
+    unsigned x;
+    std::memcpy(&x, &f, 4);
+    return x;
+}
+// host fp32 arithmetic rounds to nearest even; volatile keeps each
+// operation rounded on its own (no contraction)
+inline float __fadd_rn(float a, float b) { volatile float r = a + b; return r; }
+inline float __fsub_rn(float a, float b) { volatile float r = a - b; return r; }
+inline float __fmul_rn(float a, float b) { volatile float r = a * b; return r; }
+inline float __fdiv_rn(float a, float b) { volatile float r = a / b; return r; }
+inline float __fsqrt_rn(float a) { return std::sqrt(a); }
 inline float __int_as_float(int x)
 {
     float f;
