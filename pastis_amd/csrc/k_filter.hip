@@ -22,6 +22,13 @@
 //      diagonal's overlap relative to the longer sequence
 #include "k_filter.h"
 
+// Every fp32 operation below rounds on its own, as in the CPU oracle: hipcc
+// contracts a * b + c into an FMA by default (also through __fmul_rn /
+// __fadd_rn, whose bodies come from the HIP headers), so the Makefile builds
+// this file with -ffp-contract=off.  fmaf is used where the product of two
+// bf16 values is exact, which makes the FMA equal to the separately rounded
+// multiply and add.
+
 namespace pa {
 namespace {
 
