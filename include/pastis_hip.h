@@ -150,6 +150,11 @@ const char *pa_last_error(struct pa_ctx *ctx);
 
 /* ---- sequences ------------------------------------------------------- */
 
+/* Coverage and identity thresholds of the predicate for later calls; the
+ * reference passes them with every batch (params.aln_cov_thr, aln_ani_thr,
+ * SeqanFullAligner.cpp:253-256). */
+int pa_set_thresholds(struct pa_ctx *ctx, double cov_thr, int ani_thr);
+
 /* Replaces construct_seqs / construct_seqs_bl
  * (src/align/SeqanFullAligner.cpp:28-168, SeqanXdropAligner.cpp:32-185):
  * uploads one sequence set once.  residues[offsets[i] .. offsets[i+1]) is

@@ -1,0 +1,45 @@
+/*
+ * pastis_pw.h -- C entry point of the C++ operator layer (libpastis_pw.so):
+ * HipAligner (a pastis::PWAlign, inc/align/PWAlign.hpp:20-82) driven by the
+ * batch loop of pw_aln_batch (src/DistPWRunner.cpp:30-160) over one
+ * diagonal block, as src/sim_search.cpp:119-135 drives SeqanFullAligner /
+ * SeqanXdropAligner.  The C++ classes themselves are in
+ * pastis_amd/host/hip_aligner.hpp and pw_runner.hpp.
+ */
+#ifndef PASTIS_PW_H
+#define PASTIS_PW_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* mode 0 = --sfa (CommonKmerLight tuples), 1 = --sxa (CommonKmerLoc).
+ * res / offs: ASCII residues of nseq sequences (one diagonal block).
+ * Nonzero i is (rows[i], cols[i]) with count[i], score[i] and, in mode 1,
+ * seeds[4i..4i+3] = first.(row, col), second.(row, col); coordinates must be
+ * unique.  ckthr / mosthr: the k-mer prune of DistPWRunner.cpp:48-54;
+ * batch_sz: params.aln_batch_sz; cov_thr / ani_thr: params.aln_cov_thr /
+ * aln_ani_thr.  filter_params (PA_FILTER_NPARAM floats, NULL = no learned
+ * filter), filter_mean / filter_istd (16 floats each) and filter_tau
+ * configure the learned prefilter.  score_aln_out[i] receives the nonzero's
+ * final score_aln (0 when pruned).  stats (8 values, may be NULL): nnz in,
+ * after the k-mer prune, after the alignment prune, batches, pairs aligned,
+ * pairs dropped by the filter, pairs passed, pairs handed to aln_batch.
+ * Returns a PA_* status of include/pastis_hip.h. */
+int pw_run_block(int mode, int gap_open, int gap_ext, int klength, int xdrop,
+                 int seed_count, int gap_model, int device, const char *res,
+                 const uint64_t *offs, uint64_t nseq, uint64_t nnz,
+                 const uint64_t *rows, const uint64_t *cols,
+                 const uint16_t *count, const int16_t *score,
+                 const uint16_t *seeds, int ckthr, float mosthr,
+                 uint64_t batch_sz, double cov_thr, int ani_thr,
+                 const float *filter_params, const float *filter_mean,
+                 const float *filter_istd, float filter_tau,
+                 float *score_aln_out, uint64_t *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PASTIS_PW_H */

@@ -29,7 +29,7 @@ EXPORTS = [
     "pa_get_kernel_stats", "pa_filter_features", "pa_filter_features_dev",
     "pa_filter_set_model", "pa_filter_get_model", "pa_filter_grad",
     "pa_filter_train", "pa_filter_logits", "pa_filter_score_dev",
-    "pa_filter_select_dev",
+    "pa_filter_select_dev", "pa_set_thresholds",
 ]
 
 PA_FILTER_NF = 16
@@ -126,6 +126,7 @@ def lib():
     L.pa_reset_stats.argtypes = [vp]
     L.pa_get_kernel_stats.argtypes = [vp, C.POINTER(PaKernelStat), i32,
                                       C.POINTER(C.c_int)]
+    L.pa_set_thresholds.argtypes = [vp, C.c_double, i32]
     fp = C.POINTER(C.c_float)
     L.pa_filter_features.argtypes = [vp, vp, u64, vp]
     L.pa_filter_features_dev.argtypes = [vp, vp, u64, vp, vp]

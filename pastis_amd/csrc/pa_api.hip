@@ -700,6 +700,17 @@ void pa_destroy(pa_ctx *c)
     delete c;
 }
 
+int pa_set_thresholds(pa_ctx *c, double cov_thr, int ani_thr)
+{
+    if (!c)
+        return PA_ERR_ARG;
+    if (!(cov_thr >= 0.0 && cov_thr <= 1e9))
+        return fail(c, PA_ERR_ARG, "coverage threshold must be >= 0");
+    c->cfg.cov_thr = cov_thr;
+    c->cfg.ani_thr = ani_thr;
+    return PA_OK;
+}
+
 int pa_load_seqs(pa_ctx *c, int side, const uint8_t *residues,
                  const uint64_t *offsets, uint64_t nseq, int encoding)
 {

@@ -28,6 +28,17 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(L, name), name
 
 
+def test_operator_layer_exports_its_entry_point():
+    text = open(os.path.join(ROOT, "include", "pastis_pw.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    decl = sorted(set(re.findall(r"\b(pw_[a-z0-9_]+)\s*\(", text)))
+    assert decl == ["pw_run_block"]
+    from pastis_amd import pwalign
+    L = pwalign.lib()
+    for name in decl:
+        assert hasattr(L, name), name
+
+
 def test_strerror_names_every_status():
     L = _lib.lib()
     for code in range(11):
