@@ -62,7 +62,7 @@ def roofline(kstats, st, step_ms_total):
                           peak=PEAK_I16_TOPS if k["name"].endswith("i16")
                           else PEAK_I32_TOPS))
     if st.get("ms_xdrop", 0) > 0:
-        cands.append(dict(kernel="k_xdrop_ring", ms=st["ms_xdrop"],
+        cands.append(dict(kernel="k_xdrop_lds", ms=st["ms_xdrop"],
                           launches=None, cells=st["xdrop_cells"], ops=7,
                           peak=PEAK_I32_TOPS))
     if not cands:
@@ -136,6 +136,83 @@ def cpu_baseline_sw(w, pairs, go, ge, budget_s, threads):
                   (n, len(pairs), dt, cells)}
 
 
+def learned_filter(args, cfg, al, d_pairs, n, plain_out, plain_ms):
+    """Train the prefilter on GPU-aligner labels of a separate config-2
+    workload (2M pairs at scale 1, generator seed 202), calibrate tau for
+    99.5 % recall on a held-out tenth, then run filter + exact aligner on the
+    benchmark pairs and compare with the plain step."""
+    from pastis_amd import prefilter, synth
+    from pastis_amd._lib import PAIR_DTYPE, RESULT_DTYPE
+    from pastis_amd.aligner import Aligner
+    t = time.time()
+    tw = synth.config2(scale=0.4 * args.scale, seed=202, threads=host_cores())
+    tal = Aligner(cfg)
+    tal.load_seqs(tw.codes, tw.offsets)
+    y = prefilter.labels_of(tal.align(tw.pairs))
+    flt = prefilter.LearnedFilter(tal, seed=0)
+    f = flt.features(tw.pairs)
+    ntr = len(y) * 9 // 10
+    t_train = time.time()
+    losses = flt.train(f[:ntr], y[:ntr], epochs=4, batch=4096, lr=2e-3)
+    t_train = time.time() - t_train
+    tau = flt.calibrate(f[ntr:], y[ntr:], recall=0.995)
+    params, mean, istd = tal.filter_get_model()
+    tal.close()
+    log("filter: trained on %d pairs (%d positive) in %.2f s, losses %s, "
+        "tau %.4f (setup %.1f s)" % (ntr, int(y[:ntr].sum()), t_train,
+                                     np.round(losses, 4).tolist(), tau,
+                                     time.time() - t))
+    # evaluation on the benchmark pairs against the exact labels
+    al.filter_set_model(params, mean, istd)
+    d_log = al.device_alloc(4 * n)
+    al.filter_score_dev(d_pairs, n, d_log)
+    z = np.zeros(n, np.float32)
+    al.synchronize()
+    al.d2h(z, d_log)
+    al.device_free(d_log)
+    labels = prefilter.labels_of(plain_out)
+    rep = prefilter.report(z, labels, tau)
+    # timed: select + exact alignment of the kept pairs
+    d_kept = al.device_alloc(n * PAIR_DTYPE.itemsize)
+    d_idx = al.device_alloc(4 * n)
+    d_res = al.device_alloc(n * RESULT_DTYPE.itemsize)
+
+    def fstep():
+        k = al.filter_select_dev(d_pairs, n, float(tau), d_kept, d_idx)
+        al.align_dev(d_kept, k, d_res)
+        return k
+
+    for _ in range(args.warmup):
+        fstep()
+    al.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        k = fstep()
+    al.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    res = np.zeros(k, RESULT_DTYPE)
+    idx = np.zeros(k, np.uint32)
+    al.d2h(res, d_res)
+    al.d2h(idx, d_idx)
+    exact = bool(np.array_equal(res, plain_out[idx]))
+    for b in (d_kept, d_idx, d_res):
+        al.device_free(b)
+    return {
+        "model": "MLP 16-64-64-1, bf16 weights/activations, fp32 accumulate",
+        "train_pairs": int(ntr), "train_seconds": round(t_train, 3),
+        "epochs": 4, "final_loss": round(float(losses[-1]), 5),
+        "tau_logit": round(float(tau), 5),
+        "recall": round(rep["recall"], 5),
+        "precision": round(rep["precision"], 5),
+        "kept_frac": round(rep["kept_frac"], 5),
+        "band_frac": round(rep["band_frac"], 5),
+        "ms_per_step": round(ms, 3),
+        "pairs_per_s": round(n / (ms * 1e-3), 1),
+        "speedup_vs_plain": round(plain_ms / ms, 3),
+        "kept_results_identical_to_plain": exact,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -147,6 +224,8 @@ def main():
                     choices=["as_called", "as_documented"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-filter", action="store_true",
+                    help="skip the learned-filter vs plain comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU "
@@ -228,6 +307,9 @@ def main():
                                ("ms_sw", "ms_xdrop", "ms_nw", "ms_trace")},
         "cpu_baseline": None,
     }
+    if args.workload == "xdrop" and not args.no_filter:
+        line["learned_filter"] = learned_filter(args, cfg, al, d_pairs, n, out,
+                                                ms_per_step)
     if not args.no_cpu:
         threads = host_cores()
         if args.workload == "xdrop":
