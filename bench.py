@@ -224,9 +224,24 @@ def main():
                     choices=["as_called", "as_documented"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=float, default=0.0, metavar="GIB",
+                    help="only run 3 device-to-device copies of GIB GiB "
+                         "(HBM counter calibration) and exit")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shapes")
     ap.add_argument("--no-filter", action="store_true",
                     help="skip the learned-filter vs plain comparison")
     args = ap.parse_args()
+    if args.copy_probe > 0:
+        import torch
+        nb = int(args.copy_probe * (1 << 30))
+        a = torch.empty(nb, dtype=torch.uint8, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("copied %d bytes 3 times" % nb)
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU "
               "path (pairs shard by index; see DESIGN.md)", file=sys.stderr)
@@ -254,6 +269,15 @@ def main():
     out_dtype = RESULT_DTYPE if args.workload == "xdrop" else SCORE_DTYPE
     d_out = al.device_alloc(n * out_dtype.itemsize)
     al.h2d(d_pairs, pairs)
+    tune_table = []
+    if not args.no_autotune:
+        from pastis_amd import autotune
+        t = time.time()
+        choice, tune_table = autotune.tune(al, pairs,
+                                           xdrop=args.workload == "xdrop",
+                                           log=log)
+        log("autotune picked %s in %.1f s" % (choice, time.time() - t))
+    tuning = al.tuning()
 
     def step():
         if args.workload == "xdrop":
@@ -296,7 +320,8 @@ def main():
                    "pairs": n, "gap_model": args.gap_model,
                    "scale": args.scale,
                    "xdrop": 49 if args.workload == "xdrop" else None,
-                   "seed_count": 2, "k": 6, "matrix": "BLOSUM62"},
+                   "seed_count": 2, "k": 6, "matrix": "BLOSUM62",
+                   "tuning": tuning},
         "pairs_per_s": round(n * args.steps / elapsed, 1),
         "cells_per_step": int(cells / args.steps),
         "xdrop_cells_per_step": int(st["xdrop_cells"] / args.steps),

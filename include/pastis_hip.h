@@ -271,6 +271,21 @@ typedef struct pa_kernel_stat {
     double ms;              /* summed device time of the class's launches */
 } pa_kernel_stat;
 
+/* ---- tuning ---------------------------------------------------------
+ * Launch shapes chosen per context (no counterpart in the reference; the
+ * autotuner of pastis_amd/autotune.py searches them).  Results never depend
+ * on them. */
+typedef struct pa_tuning {
+    int32_t dp_wg_per_cu;   /* persistent DP workgroups (one wave) per CU,
+                               1..32; 0 = as many as occupancy allows */
+    int32_t xd_wg_per_cu;   /* X-drop workgroups (one wave) per CU, 0 = max */
+    int32_t xd_ring;        /* X-drop LDS ring slots per lane: 16 or 32 */
+    int32_t reserved;
+} pa_tuning;
+
+int pa_get_tuning(struct pa_ctx *ctx, pa_tuning *out);
+int pa_set_tuning(struct pa_ctx *ctx, const pa_tuning *t);
+
 /* ---- counters --------------------------------------------------------- */
 
 /* Timers and counters of the calls made on a context, the device-side

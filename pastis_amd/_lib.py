@@ -29,7 +29,8 @@ EXPORTS = [
     "pa_get_kernel_stats", "pa_filter_features", "pa_filter_features_dev",
     "pa_filter_set_model", "pa_filter_get_model", "pa_filter_grad",
     "pa_filter_train", "pa_filter_logits", "pa_filter_score_dev",
-    "pa_filter_select_dev", "pa_set_thresholds",
+    "pa_filter_select_dev", "pa_set_thresholds", "pa_get_tuning",
+    "pa_set_tuning",
 ]
 
 PA_FILTER_NF = 16
@@ -71,6 +72,14 @@ class PaStats(C.Structure):
 class PaKernelStat(C.Structure):
     _fields_ = [("name", C.c_char * 24), ("launches", C.c_uint64),
                 ("cells", C.c_uint64), ("ms", C.c_double)]
+
+
+class PaTuning(C.Structure):
+    _fields_ = [("dp_wg_per_cu", C.c_int32), ("xd_wg_per_cu", C.c_int32),
+                ("xd_ring", C.c_int32), ("reserved", C.c_int32)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_ if n != "reserved"}
 
 
 class PaTrainConfig(C.Structure):
@@ -127,6 +136,8 @@ def lib():
     L.pa_get_kernel_stats.argtypes = [vp, C.POINTER(PaKernelStat), i32,
                                       C.POINTER(C.c_int)]
     L.pa_set_thresholds.argtypes = [vp, C.c_double, i32]
+    L.pa_get_tuning.argtypes = [vp, C.POINTER(PaTuning)]
+    L.pa_set_tuning.argtypes = [vp, C.POINTER(PaTuning)]
     fp = C.POINTER(C.c_float)
     L.pa_filter_features.argtypes = [vp, vp, u64, vp]
     L.pa_filter_features_dev.argtypes = [vp, vp, u64, vp, vp]

@@ -136,6 +136,21 @@ class Aligner:
     def reset_stats(self):
         check(self._ctx, lib().pa_reset_stats(self._ctx), "pa_reset_stats")
 
+    def tuning(self):
+        t = _lib.PaTuning()
+        check(self._ctx, lib().pa_get_tuning(self._ctx, C.byref(t)),
+              "pa_get_tuning")
+        return t.as_dict()
+
+    def set_tuning(self, **kw):
+        t = _lib.PaTuning()
+        check(self._ctx, lib().pa_get_tuning(self._ctx, C.byref(t)),
+              "pa_get_tuning")
+        for k, v in kw.items():
+            setattr(t, k, int(v))
+        check(self._ctx, lib().pa_set_tuning(self._ctx, C.byref(t)),
+              "pa_set_tuning")
+
     # ---- learned prefilter ------------------------------------------------
     def filter_features(self, pairs):
         pairs = np.ascontiguousarray(pairs, PAIR_DTYPE)

@@ -406,15 +406,41 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
             uint64_t rec = TRACE ? trace_rec(qlen, tlen, R, p, 1, lane) : 0;
             const uint32_t rstep = TRACE ? (p + 1 < npass ? 64u : (uint32_t)nact)
                                          : 0u;
+            // Column residues and (in later passes) the boundary row reach
+            // lane 0 through v_readlane from registers filled 64 columns at a
+            // time: lane l holds column base + l, and the next block is
+            // loaded one block ahead, so no step waits on global memory.
+            // The boundary row of this pass is written 128 steps after it is
+            // read, so the look-ahead never reads a value of this pass.
+            auto res_at = [&](int j) {   // residue of column j (1-based)
+                return j <= tlen ? (int)ts[j - 1] : kSentinel;
+            };
+            int rc_cur = res_at(1 + lane), rc_nxt = res_at(65 + lane);
+            int bh_cur = 0, bf_cur = NEG, bh_nxt = 0, bf_nxt = NEG;
+            if (p > 0) {
+                // boundary block k holds columns 64k .. 64k+63
+                if (lane <= tlen) {
+                    bh_cur = bnd[2 * lane];
+                    bf_cur = bnd[2 * lane + 1];
+                }
+                if (64 + lane <= tlen) {
+                    bh_nxt = bnd[2 * (64 + lane)];
+                    bf_nxt = bnd[2 * (64 + lane) + 1];
+                }
+            }
             for (int t = 1; t <= last_step; ++t, rec += rstep) {
-                // column residues: lane 0 reads, the others take lane-1's
-                // residue of the previous step (two columns back for them)
-                // (no lane-dependent branches around the DPP moves: lane 0's
-                // inputs are selected, every lane loads the same address)
+                // column residues: lane 0 takes column t, the others take
+                // lane-1's residue of the previous step (two columns back for
+                // them); no lane-dependent branch surrounds the DPP move
                 {
+                    const int ri = (t - 1) & 63;
+                    if (ri == 0 && t > 1) {
+                        rc_cur = rc_nxt;
+                        rc_nxt = res_at(t + 64 + lane);
+                    }
                     const int nc = __builtin_amdgcn_update_dpp(
                         0, c_hi, 0x138, 0xf, 0xf, false);
-                    const int c0 = t <= tlen ? (int)ts[t - 1] : kSentinel;
+                    const int c0 = __builtin_amdgcn_readlane(rc_cur, ri);
                     c_hi = c_lo;
                     c_lo = lane == 0 ? c0 : nc;
                 }
@@ -423,9 +449,19 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
                     const int jn = t + 1;
                     if (p == 0) {
                         nbh = LOCAL ? 0 : open + (jn - 1) * gmax;
-                    } else if (jn <= tlen) {
-                        nbh = bnd[2 * jn];
-                        nbf = bnd[2 * jn + 1];
+                    } else {
+                        const int bi = jn & 63;
+                        if (bi == 0) {
+                            bh_cur = bh_nxt;
+                            bf_cur = bf_nxt;
+                            const int jb = jn + 64 + lane;
+                            bh_nxt = jb <= tlen ? bnd[2 * jb] : 0;
+                            bf_nxt = jb <= tlen ? bnd[2 * jb + 1] : NEG;
+                        }
+                        if (jn <= tlen) {
+                            nbh = __builtin_amdgcn_readlane(bh_cur, bi);
+                            nbf = __builtin_amdgcn_readlane(bf_cur, bi);
+                        }
                     }
                 }
                 const int jlo = t - 2 * lane;
@@ -690,6 +726,30 @@ hipError_t launch_one(const DpArgs &a, int grid, hipStream_t s)
     return hipGetLastError();
 }
 
+template <int R, bool L, bool T, bool I>
+int occupancy_one()
+{
+    const size_t lds = (size_t)kProfChars * 128 * R + 1024;
+    int n = 0;
+    hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_dp<R, L, T, I>, 64,
+                                                 lds);
+    return n > 0 ? n : 1;
+}
+
+template <int R>
+int occupancy_r(bool local, bool trace, bool i16)
+{
+    if (local) {
+        if (trace)
+            return i16 ? occupancy_one<R, true, true, true>()
+                       : occupancy_one<R, true, true, false>();
+        return i16 ? occupancy_one<R, true, false, true>()
+                   : occupancy_one<R, true, false, false>();
+    }
+    return i16 ? occupancy_one<R, false, true, true>()
+               : occupancy_one<R, false, true, false>();
+}
+
 template <int R>
 hipError_t launch_r(bool local, bool trace, bool i16, const DpArgs &a,
                     int grid, hipStream_t s)
@@ -716,6 +776,18 @@ hipError_t launch_dp(int R, bool local, bool trace, bool i16, const DpArgs &a,
     case 8: return launch_r<8>(local, trace, i16, a, grid, s);
     default: return hipErrorInvalidValue;
     }
+}
+
+int dp_occupancy(int R, bool local, bool trace, bool i16)
+{
+    static int cache[3][8] = {};
+    const int ri = R == 2 ? 0 : (R == 4 ? 1 : 2);
+    const int vi = (local ? 4 : 0) + (trace ? 2 : 0) + (i16 ? 1 : 0);
+    if (!cache[ri][vi])
+        cache[ri][vi] = R == 2   ? occupancy_r<2>(local, trace, i16)
+                        : R == 4 ? occupancy_r<4>(local, trace, i16)
+                                 : occupancy_r<8>(local, trace, i16);
+    return cache[ri][vi];
 }
 
 hipError_t launch_traceback(const TbArgs &a, int grid, hipStream_t s)

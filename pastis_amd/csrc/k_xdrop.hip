@@ -12,7 +12,8 @@
 // by absolute column (slot = column mod ring size).  The algorithm only reads
 // columns inside the range written on that anti-diagonal ([off, maxCol]), so
 // the ring is exact while that range fits; a wider range reports overflow.
-//   - k_xdrop_lds: 32-slot rings in LDS ([3][32][64 lanes] int32);
+//   - k_xdrop_lds<RING>: RING-slot rings in LDS ([3][RING][64 lanes] int32,
+//     RING = 16 or 32: fewer slots leave room for more waves per CU);
 //   - k_xdrop_wide: rings in global scratch sized to the longest sequence
 //     (the overflow path).
 // Scheduling: a first pass runs with an anti-diagonal budget; items still
@@ -26,7 +27,6 @@ namespace pa {
 namespace {
 
 constexpr int32_t kUndef = INT_MIN / 4;
-constexpr int kRing = 32;
 
 enum : uint16_t { kXdOvf = 1, kXdLong = 2 };
 
@@ -222,16 +222,17 @@ struct XdArgs {
     uint32_t *ovf_list, *ovf_count, *long_list, *long_count;
 };
 
+template <int RING>
 __global__ __launch_bounds__(64) void k_xdrop_lds(XdArgs a)
 {
     __shared__ int8_t mat[32 * 32];
-    __shared__ int32_t ring[3 * kRing * 64];
+    __shared__ int32_t ring[3 * RING * 64];
     for (int i = threadIdx.x; i < 32 * 32; i += blockDim.x)
         mat[i] = a.mat[i];
     __syncthreads();
     const uint32_t n = *a.count;
     const int lane = threadIdx.x;
-    const Rings A{ring + lane, kRing * 64, 64, kRing - 1, kRing};
+    const Rings A{ring + lane, RING * 64, 64, RING - 1, RING};
     for (;;) {
         // all lanes join the atomic (lane 0 adds 64) and no divergent
         // continue follows, so the wave re-enters the loop together
@@ -285,15 +286,36 @@ hipError_t launch_xdrop(const XdTask *tasks, const uint32_t *list,
                         int budget, int32_t *scratch, int cap, XdOut *out,
                         uint32_t *ovf_list, uint32_t *ovf_count,
                         uint32_t *long_list, uint32_t *long_count, int grid,
-                        bool wide, hipStream_t s)
+                        bool wide, int ring, hipStream_t s)
 {
     XdArgs a{tasks, list, count, queue, mat, gap, xdrop, seed_len, budget,
              scratch, cap, out, ovf_list, ovf_count, long_list, long_count};
     if (wide)
         hipLaunchKernelGGL(k_xdrop_wide, dim3(grid), dim3(256), 0, s, a);
+    else if (ring == 16)
+        hipLaunchKernelGGL(k_xdrop_lds<16>, dim3(grid), dim3(64), 0, s, a);
+    else if (ring == 32)
+        hipLaunchKernelGGL(k_xdrop_lds<32>, dim3(grid), dim3(64), 0, s, a);
     else
-        hipLaunchKernelGGL(k_xdrop_lds, dim3(grid), dim3(64), 0, s, a);
+        return hipErrorInvalidValue;
     return hipGetLastError();
+}
+
+int xdrop_occupancy(int ring)
+{
+    static int cache[2] = {0, 0};
+    const int i = ring == 16 ? 0 : 1;
+    if (!cache[i]) {
+        int n = 0;
+        if (ring == 16)
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_xdrop_lds<16>,
+                                                         64, 0);
+        else
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_xdrop_lds<32>,
+                                                         64, 0);
+        cache[i] = n > 0 ? n : 1;
+    }
+    return cache[i];
 }
 
 }  // namespace pa

@@ -86,8 +86,7 @@ enum {
     C_N = 32
 };
 
-constexpr int kDpGrid = 2048;      // persistent DP workgroups (one wave each)
-constexpr int kXdRingBlocks = 4096;  // X-drop LDS-ring workgroups (one wave)
+constexpr int kMaxWgPerCu = 32;     // one-wave workgroups per CU (8 per SIMD)
 constexpr int kXdBudget = 0;         // first-pass anti-diagonal budget (0: none)
 constexpr int kXdWideLanes = 65536;  // lanes of the wide (global scratch) re-run
 
@@ -172,10 +171,34 @@ int ensure_common(pa_ctx *c, uint64_t nt)
     return PA_OK;
 }
 
+// Persistent DP grid: occupancy-limited workgroups per CU, capped by the
+// tuning and by the number of tasks.
+int dp_grid(pa_ctx *c, uint64_t ntasks, int R, bool local, bool trace,
+            bool i16)
+{
+    int per_cu = dp_occupancy(R, local, trace, i16);
+    if (c->tune.dp_wg_per_cu > 0)
+        per_cu = std::min(per_cu, (int)c->tune.dp_wg_per_cu);
+    per_cu = std::min(per_cu, kMaxWgPerCu);
+    return (int)std::max<uint64_t>(
+        1, std::min<uint64_t>(ntasks, (uint64_t)c->ncu * per_cu));
+}
+
+int xd_grid(pa_ctx *c, uint64_t nitems)
+{
+    int per_cu = xdrop_occupancy(c->tune.xd_ring);
+    if (c->tune.xd_wg_per_cu > 0)
+        per_cu = std::min(per_cu, (int)c->tune.xd_wg_per_cu);
+    per_cu = std::min(per_cu, kMaxWgPerCu);
+    return (int)std::max<uint64_t>(
+        1, std::min<uint64_t>((nitems + 63) / 64, (uint64_t)c->ncu * per_cu));
+}
+
 int ensure_bnd(pa_ctx *c)
 {
     const size_t need =
-        (size_t)kDpGrid * 2u * (size_t)(c->max_clen + 2) * sizeof(int32_t);
+        (size_t)c->ncu * kMaxWgPerCu * 2u * (size_t)(c->max_clen + 2) *
+        sizeof(int32_t);
     return grow(c, c->bnd, need);
 }
 
@@ -274,7 +297,6 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
         args.bnd_stride = 2u * (c->max_clen + 2);
         args.gap_open = c->cfg.gap_open;
         args.gap_ext = c->cfg.gap_ext;
-        const int grid = (int)std::min<uint64_t>(b - a, kDpGrid);
         {
             Scope sc(c, kind, s);
             for (int rcl = 0; rcl < 3; ++rcl) {
@@ -289,7 +311,9 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
                     {
                         Scope k(c, 100 + dmode * 16 + bin, s);
                         HIPCHK(c, launch_dp(R, local, trace, w == 0, args,
-                                            grid, s));
+                                            dp_grid(c, b - a, R, local, trace,
+                                                    w == 0),
+                                            s));
                     }
                     c->stats.launches++;
                 }
@@ -301,7 +325,9 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
                     {
                         Scope k(c, 100 + dmode * 16 + kBins + rcl, s);
                         HIPCHK(c, launch_dp(R, local, trace, false, args,
-                                            grid, s));
+                                            dp_grid(c, b - a, R, local, trace,
+                                                    false),
+                                            s));
                     }
                     HIPCHK(c, mark_flags(args.list, args.count, b - a,
                                          P<uint8_t>(c->tflags), 1, s));
@@ -408,21 +434,23 @@ int run_xdrop(pa_ctx *c, const pa_pair *d_pairs, uint64_t n, hipStream_t s)
                              P<uint64_t>(c->coffs), P<XdTask>(c->xdt), s));
     {
         Scope t(c, 1, s);
-        const int grid = (int)std::min<uint64_t>((ni + 63) / 64, kXdRingBlocks);
+        const int grid = xd_grid(c, ni);
         // pass 1: anti-diagonal budget; pass 2: the items that outlived it
         HIPCHK(c, launch_xdrop(P<XdTask>(c->xdt), nullptr, ctr + C_XDN,
                                ctr + C_XDQ, P<int8_t>(c->dmat),
                                c->cfg.xdrop_gap, c->cfg.xdrop,
                                c->cfg.seed_len, kXdBudget, nullptr, 0,
                                P<XdOut>(c->xdo), ovf, ctr + C_XDOVF, lng,
-                               ctr + C_XDLONG, grid, false, s));
+                               ctr + C_XDLONG, grid, false, c->tune.xd_ring,
+                               s));
         if (kXdBudget > 0) {
             HIPCHK(c, launch_xdrop(P<XdTask>(c->xdt), lng, ctr + C_XDLONG,
                                    ctr + C_XDLONGQ, P<int8_t>(c->dmat),
                                    c->cfg.xdrop_gap, c->cfg.xdrop,
                                    c->cfg.seed_len, 0, nullptr, 0,
                                    P<XdOut>(c->xdo), ovf, ctr + C_XDOVF, lng,
-                                   ctr + C_XDDUMMY, grid, false, s));
+                                   ctr + C_XDDUMMY, grid, false,
+                                   c->tune.xd_ring, s));
             c->stats.launches++;
         }
         c->stats.launches++;
@@ -451,7 +479,7 @@ int run_xdrop(pa_ctx *c, const pa_pair *d_pairs, uint64_t n, hipStream_t s)
                                    c->cfg.seed_len, 0, P<int32_t>(wide), cap,
                                    P<XdOut>(c->xdo), lng, ctr + C_XDDUMMY,
                                    lng, ctr + C_XDDUMMY, (int)blocks2, true,
-                                   s));
+                                   0, s));
             c->stats.launches++;
         }
         HIPCHK(c, hipStreamSynchronize(s));
@@ -642,6 +670,7 @@ int pa_create(const pa_config *cfg, pa_ctx **out)
             c->maxs = std::max<int>(c->maxs, v);
         }
     c->cfg.matrix = nullptr;
+    c->ncu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     if (hipSetDevice(c->device) != hipSuccess ||
         hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
         delete c;
@@ -698,6 +727,27 @@ void pa_destroy(pa_ctx *c)
     if (c->st)
         hipStreamDestroy(c->st);
     delete c;
+}
+
+int pa_get_tuning(pa_ctx *c, pa_tuning *out)
+{
+    if (!c || !out)
+        return PA_ERR_ARG;
+    *out = c->tune;
+    return PA_OK;
+}
+
+int pa_set_tuning(pa_ctx *c, const pa_tuning *t)
+{
+    if (!c || !t)
+        return PA_ERR_ARG;
+    if (t->dp_wg_per_cu < 0 || t->dp_wg_per_cu > kMaxWgPerCu ||
+        t->xd_wg_per_cu < 0 || t->xd_wg_per_cu > kMaxWgPerCu ||
+        (t->xd_ring != 16 && t->xd_ring != 32))
+        return fail(c, PA_ERR_ARG, "tuning outside its range");
+    c->tune = *t;
+    c->tune.reserved = 0;
+    return PA_OK;
 }
 
 int pa_set_thresholds(pa_ctx *c, double cov_thr, int ani_thr)

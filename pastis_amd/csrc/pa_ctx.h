@@ -54,6 +54,8 @@ struct pa_ctx {
     std::vector<hipEvent_t> free_events;
     pa_stats stats{};
     pa::FilterState *flt = nullptr;
+    int ncu = 256;                // compute units of the device
+    pa_tuning tune{0, 0, 16, 0};
 };
 
 namespace pa {

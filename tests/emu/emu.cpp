@@ -139,6 +139,7 @@ hipError_t hipGetDeviceCount(int *n)
 hipError_t hipGetDeviceProperties(hipDeviceProp_t *p, int)
 {
     std::strcpy(p->gcnArchName, "gfx950:sramecc+:xnack-");
+    p->multiProcessorCount = 2;
     return hipSuccess;
 }
 hipError_t hipSetDevice(int) { return hipSuccess; }

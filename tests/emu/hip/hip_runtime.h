@@ -56,6 +56,7 @@ enum hipMemcpyKind {
 constexpr unsigned hipStreamNonBlocking = 1;
 struct hipDeviceProp_t {
     char gcnArchName[256];
+    int multiProcessorCount;
 };
 
 namespace emu {
@@ -158,6 +159,11 @@ inline unsigned emu_readfirstlane(unsigned v)
     return (unsigned)emu::wave_exchange(v, 0, false);
 }
 #define __builtin_amdgcn_readfirstlane emu_readfirstlane
+inline int emu_readlane(int v, int idx)
+{
+    return (int)(uint32_t)emu::wave_exchange((uint32_t)v, idx & 63, false);
+}
+#define __builtin_amdgcn_readlane emu_readlane
 inline unsigned emu_perm(unsigned s0, unsigned s1, unsigned sel)
 {
     const uint64_t src = ((uint64_t)s0 << 32) | s1;
@@ -202,6 +208,12 @@ hipError_t hipEventElapsedTime(float *, hipEvent_t, hipEvent_t);
 hipError_t hipEventDestroy(hipEvent_t);
 hipError_t hipGetDeviceCount(int *);
 hipError_t hipGetDeviceProperties(hipDeviceProp_t *, int);
+template <class F>
+hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int *n, F, int, size_t)
+{
+    *n = 2;
+    return hipSuccess;
+}
 hipError_t hipSetDevice(int);
 hipError_t hipMemGetInfo(size_t *, size_t *);
 hipError_t hipGetLastError();

@@ -96,3 +96,22 @@ def test_seed_out_of_range_is_an_error():
     al.load_seqs(codes, offs)
     with pytest.raises(_lib.PaError):
         al.align(pairs_of([(0, 1)], [[15, 0, 0, 0]]))
+
+
+def test_results_do_not_depend_on_tuning():
+    """Launch shapes (ring slots, workgroups per CU) change speed only."""
+    w = small_config2(n_pairs=800, seed=17)
+    base = None
+    for ring, dp, xd in [(16, 0, 0), (32, 0, 0), (16, 1, 1), (32, 3, 2)]:
+        al = make_aligner("as_called")
+        al.set_tuning(xd_ring=ring, dp_wg_per_cu=dp, xd_wg_per_cu=xd)
+        assert al.tuning() == dict(dp_wg_per_cu=dp, xd_wg_per_cu=xd,
+                                   xd_ring=ring)
+        al.load_seqs(w.codes, w.offsets)
+        got = al.align(w.pairs)
+        if base is None:
+            base = got
+        assert np.array_equal(got, base), (ring, dp, xd)
+    al = make_aligner("as_called")
+    with pytest.raises(_lib.PaError):
+        al.set_tuning(xd_ring=24)
