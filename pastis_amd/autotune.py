@@ -20,10 +20,19 @@ XD_RING = (16, 32)
 
 
 def candidates(mode_xdrop=True):
-    rings = XD_RING if mode_xdrop else (16,)
-    xds = XD_WG if mode_xdrop else (0,)
-    for dp, xd, ring in itertools.product(DP_WG, xds, rings):
-        yield dict(dp_wg_per_cu=dp, xd_wg_per_cu=xd, xd_ring=ring)
+    """16-lanes-per-item X-drop with each DP grid and overflow ring size;
+    one-lane-per-item X-drop with its grid and ring choices."""
+    if not mode_xdrop:
+        for dp in DP_WG:
+            yield dict(dp_wg_per_cu=dp, xd_wg_per_cu=0, xd_ring=16,
+                       xd_perlane=0)
+        return
+    for dp, ring in itertools.product(DP_WG, XD_RING):
+        yield dict(dp_wg_per_cu=dp, xd_wg_per_cu=0, xd_ring=ring,
+                   xd_perlane=0)
+    for xd, ring in itertools.product(XD_WG, XD_RING):
+        yield dict(dp_wg_per_cu=0, xd_wg_per_cu=xd, xd_ring=ring,
+                   xd_perlane=1)
 
 
 def tune(al, pairs, xdrop=True, sample=200000, repeats=2, seed=0,
@@ -67,7 +76,8 @@ def tune(al, pairs, xdrop=True, sample=200000, repeats=2, seed=0,
         al.device_free(dp)
         al.device_free(do)
     best = min(table, key=lambda r: r["ms"])
-    choice = {k: best[k] for k in ("dp_wg_per_cu", "xd_wg_per_cu", "xd_ring")}
+    choice = {k: best[k] for k in ("dp_wg_per_cu", "xd_wg_per_cu", "xd_ring",
+                                   "xd_perlane")}
     al.set_tuning(**choice)
     if cache:
         known = {}

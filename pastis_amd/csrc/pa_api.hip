@@ -87,7 +87,6 @@ enum {
 };
 
 constexpr int kMaxWgPerCu = 32;     // one-wave workgroups per CU (8 per SIMD)
-constexpr int kXdBudget = 0;         // first-pass anti-diagonal budget (0: none)
 constexpr int kXdWideLanes = 65536;  // lanes of the wide (global scratch) re-run
 
 }  // namespace
@@ -435,25 +434,36 @@ int run_xdrop(pa_ctx *c, const pa_pair *d_pairs, uint64_t n, hipStream_t s)
     {
         Scope t(c, 1, s);
         const int grid = xd_grid(c, ni);
-        // pass 1: anti-diagonal budget; pass 2: the items that outlived it
-        HIPCHK(c, launch_xdrop(P<XdTask>(c->xdt), nullptr, ctr + C_XDN,
-                               ctr + C_XDQ, P<int8_t>(c->dmat),
-                               c->cfg.xdrop_gap, c->cfg.xdrop,
-                               c->cfg.seed_len, kXdBudget, nullptr, 0,
-                               P<XdOut>(c->xdo), ovf, ctr + C_XDOVF, lng,
-                               ctr + C_XDLONG, grid, false, c->tune.xd_ring,
-                               s));
-        if (kXdBudget > 0) {
+        if (!c->tune.xd_perlane) {
+            // 16 lanes per item first; items with a wider band go to the
+            // one-lane-per-item kernel through the `lng` list
+            const int cg = (int)std::max<uint64_t>(
+                1, std::min<uint64_t>((ni + 3) / 4,
+                                      (uint64_t)c->ncu *
+                                          std::min(xdrop_coop_occupancy(),
+                                                   kMaxWgPerCu / 4)));
+            HIPCHK(c, launch_xdrop_coop(P<XdTask>(c->xdt), nullptr,
+                                        ctr + C_XDN, ctr + C_XDQ,
+                                        P<int8_t>(c->dmat), c->cfg.xdrop_gap,
+                                        c->cfg.xdrop, c->cfg.seed_len,
+                                        P<XdOut>(c->xdo), lng, ctr + C_XDLONG,
+                                        cg, s));
             HIPCHK(c, launch_xdrop(P<XdTask>(c->xdt), lng, ctr + C_XDLONG,
                                    ctr + C_XDLONGQ, P<int8_t>(c->dmat),
                                    c->cfg.xdrop_gap, c->cfg.xdrop,
-                                   c->cfg.seed_len, 0, nullptr, 0,
-                                   P<XdOut>(c->xdo), ovf, ctr + C_XDOVF, lng,
-                                   ctr + C_XDDUMMY, grid, false,
+                                   c->cfg.seed_len, nullptr, 0,
+                                   P<XdOut>(c->xdo), ovf, ctr + C_XDOVF, grid,
+                                   c->tune.xd_ring, s));
+            c->stats.launches += 2;
+        } else {
+            HIPCHK(c, launch_xdrop(P<XdTask>(c->xdt), nullptr, ctr + C_XDN,
+                                   ctr + C_XDQ, P<int8_t>(c->dmat),
+                                   c->cfg.xdrop_gap, c->cfg.xdrop,
+                                   c->cfg.seed_len, nullptr, 0,
+                                   P<XdOut>(c->xdo), ovf, ctr + C_XDOVF, grid,
                                    c->tune.xd_ring, s));
             c->stats.launches++;
         }
-        c->stats.launches++;
     }
     uint32_t novf = 0;
     HIPCHK(c, hipMemcpyAsync(&novf, ctr + C_XDOVF, 4, hipMemcpyDeviceToHost, s));
@@ -476,10 +486,9 @@ int run_xdrop(pa_ctx *c, const pa_pair *d_pairs, uint64_t n, hipStream_t s)
             HIPCHK(c, launch_xdrop(P<XdTask>(c->xdt), ovf, ctr + C_XDOVF,
                                    ctr + C_XDOVFQ, P<int8_t>(c->dmat),
                                    c->cfg.xdrop_gap, c->cfg.xdrop,
-                                   c->cfg.seed_len, 0, P<int32_t>(wide), cap,
+                                   c->cfg.seed_len, P<int32_t>(wide), cap,
                                    P<XdOut>(c->xdo), lng, ctr + C_XDDUMMY,
-                                   lng, ctr + C_XDDUMMY, (int)blocks2, true,
-                                   0, s));
+                                   (int)blocks2, 0, s));
             c->stats.launches++;
         }
         HIPCHK(c, hipStreamSynchronize(s));
@@ -745,8 +754,9 @@ int pa_set_tuning(pa_ctx *c, const pa_tuning *t)
         t->xd_wg_per_cu < 0 || t->xd_wg_per_cu > kMaxWgPerCu ||
         (t->xd_ring != 16 && t->xd_ring != 32))
         return fail(c, PA_ERR_ARG, "tuning outside its range");
+    if (t->xd_perlane != 0 && t->xd_perlane != 1)
+        return fail(c, PA_ERR_ARG, "xd_perlane must be 0 or 1");
     c->tune = *t;
-    c->tune.reserved = 0;
     return PA_OK;
 }
 

@@ -135,6 +135,15 @@ static_assert(sizeof(XdOut) == 40, "XdOut layout");
 hipError_t launch_dp(int R, bool local, bool trace, bool i16,
                      const DpArgs &a, int grid, hipStream_t s);
 hipError_t launch_traceback(const TbArgs &a, int grid, hipStream_t s);
+// X-drop with one 16-lane row of a wave per work item (k_xdrop_coop.hip);
+// items whose band outgrows 16 columns are appended to ovf_list.
+hipError_t launch_xdrop_coop(const XdTask *tasks, const uint32_t *list,
+                             const uint32_t *count, uint32_t *queue,
+                             const int8_t *mat, int gap, int xdrop,
+                             int seed_len, XdOut *out, uint32_t *ovf_list,
+                             uint32_t *ovf_count, int grid, hipStream_t s);
+int xdrop_coop_occupancy();
+
 // Workgroups of one DP kernel variant that fit on one CU (occupancy).
 int dp_occupancy(int R, bool local, bool trace, bool i16);
 // Same for the LDS-ring X-drop kernel with `ring` slots per lane.
@@ -143,9 +152,8 @@ int xdrop_occupancy(int ring);
 hipError_t launch_xdrop(const XdTask *tasks, const uint32_t *list,
                         const uint32_t *count, uint32_t *queue,
                         const int8_t *mat, int gap, int xdrop, int seed_len,
-                        int budget, int32_t *scratch, int cap, XdOut *out,
-                        uint32_t *ovf_list, uint32_t *ovf_count,
-                        uint32_t *long_list, uint32_t *long_count, int grid,
-                        bool wide, int ring, hipStream_t s);
+                        int32_t *scratch, int cap, XdOut *out,
+                        uint32_t *ovf_list, uint32_t *ovf_count, int grid,
+                        int ring, hipStream_t s);
 
 }  // namespace pa

@@ -144,16 +144,25 @@ inline float __int_as_float(int x)
     return f;
 }
 
-// wave_shr:1 (control 0x138) is the only DPP pattern the kernels use.
+// DPP patterns the kernels use: wave_shr:1 (0x138) and row_ror:k
+// (0x121..0x12f, lane j of a 16-lane row reads lane (j - k) mod 16).
 inline int emu_update_dpp(int old, int src, int ctrl, int, int, bool)
 {
-    if (ctrl != 0x138)
-        std::abort();
     const int lane = (int)(emu::tIdx.x & 63);
-    uint64_t v = emu::wave_exchange((uint32_t)src, lane - 1, false);
-    return lane == 0 ? old : (int)(uint32_t)v;
+    if (ctrl == 0x138) {
+        uint64_t v = emu::wave_exchange((uint32_t)src, lane - 1, false);
+        return lane == 0 ? old : (int)(uint32_t)v;
+    }
+    if (ctrl > 0x120 && ctrl <= 0x12f) {
+        const int k = ctrl - 0x120;
+        const int srcl = (lane & ~15) | ((lane - k) & 15);
+        return (int)(uint32_t)emu::wave_exchange((uint32_t)src, srcl, false);
+    }
+    std::abort();
 }
 #define __builtin_amdgcn_update_dpp emu_update_dpp
+inline int max(int a, int b) { return a > b ? a : b; }
+inline int min(int a, int b) { return a < b ? a : b; }
 inline unsigned emu_readfirstlane(unsigned v)
 {
     return (unsigned)emu::wave_exchange(v, 0, false);

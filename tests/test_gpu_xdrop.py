@@ -102,16 +102,23 @@ def test_results_do_not_depend_on_tuning():
     """Launch shapes (ring slots, workgroups per CU) change speed only."""
     w = small_config2(n_pairs=800, seed=17)
     base = None
-    for ring, dp, xd in [(16, 0, 0), (32, 0, 0), (16, 1, 1), (32, 3, 2)]:
-        al = make_aligner("as_called")
-        al.set_tuning(xd_ring=ring, dp_wg_per_cu=dp, xd_wg_per_cu=xd)
-        assert al.tuning() == dict(dp_wg_per_cu=dp, xd_wg_per_cu=xd,
-                                   xd_ring=ring)
-        al.load_seqs(w.codes, w.offsets)
-        got = al.align(w.pairs)
-        if base is None:
-            base = got
-        assert np.array_equal(got, base), (ring, dp, xd)
+    for model in ("as_called", "as_documented"):
+        base = None
+        for ring, dp, xd, pl in [(16, 0, 0, 0), (32, 0, 0, 1), (16, 1, 1, 1),
+                                 (32, 3, 2, 0), (16, 0, 0, 1)]:
+            al = make_aligner(model)
+            al.set_tuning(xd_ring=ring, dp_wg_per_cu=dp, xd_wg_per_cu=xd,
+                          xd_perlane=pl)
+            assert al.tuning() == dict(dp_wg_per_cu=dp, xd_wg_per_cu=xd,
+                                       xd_ring=ring, xd_perlane=pl)
+            al.load_seqs(w.codes, w.offsets)
+            al.reset_stats()
+            got = al.align(w.pairs)
+            cells = al.stats()["xdrop_cells"]
+            if base is None:
+                base = (got, cells)
+            assert np.array_equal(got, base[0]), (model, ring, dp, xd, pl)
+            assert cells == base[1]
     al = make_aligner("as_called")
     with pytest.raises(_lib.PaError):
         al.set_tuning(xd_ring=24)
