@@ -294,12 +294,21 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
     const V EXT = O::splat(TRACE ? ext + 1 : ext);
     int32_t *bnd = a.bnd + (size_t)blockIdx.x * a.bnd_stride;
 
+    // tasks claimed by this wave and not yet started: [cbase, cend)
+    uint32_t cbase = 0, cend = 0;
+    const uint32_t chunk = a.chunk;
     for (;;) {
-        // Every lane takes part in the queue atomic (only lane 0 adds): a
-        // lane-0-only branch ahead of the readfirstlane lets the compiler
-        // rotate the loop so that the other lanes re-enter it without lane 0.
-        uint32_t k = atomicAdd(a.queue, lane == 0 ? 1u : 0u);
-        k = __builtin_amdgcn_readfirstlane(k);
+        // One queue atomic claims `chunk` tasks.  Every lane takes part in
+        // the atomic (only lane 0 adds): a lane-0-only branch ahead of the
+        // readfirstlane lets the compiler rotate the loop so that the other
+        // lanes re-enter it without lane 0.
+        if (cbase >= cend) {
+            uint32_t b = atomicAdd(a.queue, lane == 0 ? chunk : 0u);
+            b = __builtin_amdgcn_readfirstlane(b);
+            cbase = b;
+            cend = b + chunk;
+        }
+        const uint32_t k = cbase++;
         if (k >= count)
             break;
         const uint32_t tid = a.list[k];

@@ -50,9 +50,54 @@ struct Item {
     int off1, off2, off3, l1, l2, l3;
     int r1, r2, r3;           // ring of anti-diagonals d-2, d-1, d
     uint32_t nc;              // evaluated cells
+    // residue staging (LDS kernels): the request issued at the start of the
+    // previous anti-diagonal is stored at the start of this one
+    const uint8_t *qb, *db;   // residue of column c at qb[qstep * c], row r
+    int qstep, dstep;         // at db[dstep * r]
+    int qhi, rhi;             // highest column / row requested
+    int rq, rcol, rd, rrow;   // request in flight: residues, column, row
 };
 
-__device__ inline void item_init(Item &s, const Rings &A, int gap, int X)
+// Residues of column col (query) and row row (database) of the item.
+__device__ inline int q_at(const Item &s, int col)
+{
+    return s.qb[s.qstep * col];
+}
+__device__ inline int d_at(const Item &s, int row)
+{
+    return s.db[s.dstep * row];
+}
+
+// Per-lane LDS rings of residues, 32 slots each, indexed by column and row:
+// the band never spans more than 31 columns / rows ahead of what is staged.
+struct ResRings {
+    uint8_t *q, *d;   // slot stride 64 (one byte per lane)
+    __device__ inline uint8_t &qs(int col) const { return q[(col & 31) * 64]; }
+    __device__ inline uint8_t &ds(int row) const { return d[(row & 31) * 64]; }
+};
+
+// Stores the request in flight into the rings, then requests column `col`
+// and row `row` (one already requested, or outside the segments, is skipped:
+// the load then reads column / row 1 again and is not stored).
+__device__ inline void stage(Item &s, const ResRings &Rr, int col, int row)
+{
+    if (s.rcol)
+        Rr.qs(s.rcol) = (uint8_t)s.rq;
+    if (s.rrow)
+        Rr.ds(s.rrow) = (uint8_t)s.rd;
+    const bool qv = col > s.qhi && col < s.cols;
+    const bool dv = row > s.rhi && row < s.rows;
+    s.rcol = qv ? col : 0;
+    s.rrow = dv ? row : 0;
+    s.rq = q_at(s, qv ? col : 1);
+    s.rd = d_at(s, dv ? row : 1);
+    s.qhi = qv ? col : s.qhi;
+    s.rhi = dv ? row : s.rhi;
+}
+
+template <bool STAGE>
+__device__ inline void item_init(Item &s, const Rings &A, const ResRings &Rr,
+                                 int gap, int X)
 {
     s.r1 = 0;
     s.r2 = 1;
@@ -69,13 +114,26 @@ __device__ inline void item_init(Item &s, const Rings &A, int gap, int X)
     s.d = 1;
     s.best = 0;
     s.nc = 0;
+    if (STAGE) {
+        // anti-diagonal 2 reads column 1 and row 1
+        Rr.qs(1) = (uint8_t)q_at(s, 1);
+        Rr.ds(1) = (uint8_t)d_at(s, 1);
+        s.qhi = 1;
+        s.rhi = 1;
+        s.rcol = s.rrow = 0;
+    }
 }
 
 // One anti-diagonal.  Returns 0 while the band is not empty, 1 when the
 // extension ended, 2 when the written range outgrew the rings.
-__device__ inline int item_step(Item &s, const Rings &A, const int8_t *mat,
-                                int gap, int X)
+template <bool STAGE>
+__device__ inline int item_step(Item &s, const Rings &A, const ResRings &Rr,
+                                const int8_t *mat, int gap, int X)
 {
+    // anti-diagonal d+1 (the next one) reads columns < maxCol(d+1) <=
+    // maxCol + 1 and rows <= d + 1 - minCol(d+1) <= d + 1 - minCol
+    if (STAGE)
+        stage(s, Rr, s.maxCol, s.d + 2 - s.minCol);
     const int d = ++s.d;
     const int t = s.r1;
     s.r1 = s.r2;
@@ -114,7 +172,9 @@ __device__ inline int item_step(Item &s, const Rings &A, const int8_t *mat,
         }
         const int32_t l = A.at(r2, col - 1), u = A.at(r2, col);
         int32_t tmp = (l > u ? l : u) + gap;
-        const int32_t dg = A.at(r1, col - 1) + mat[(int)s.qs[qp] * 32 + s.ds[dp]];
+        const int qr = STAGE ? (int)Rr.qs(col) : (int)s.qs[qp];
+        const int dr = STAGE ? (int)Rr.ds(d - col) : (int)s.ds[dp];
+        const int32_t dg = A.at(r1, col - 1) + mat[qr * 32 + dr];
         tmp = dg > tmp ? dg : tmp;
         if (tmp < lim) {
             A.at(r3, col) = kUndef;
@@ -216,12 +276,14 @@ __device__ inline void item_write(const XdArgs &a, uint32_t k, int32_t score,
 }
 
 enum : int { kIdle = 0, kRun = 1, kExhausted = 2 };
+constexpr uint32_t kChunk = 64;   // items claimed per queue atomic
 
 template <int RING>
 __global__ __launch_bounds__(RING ? 64 : 256) void k_xdrop_pl(XdArgs a)
 {
     __shared__ int8_t mat[32 * 32];
     __shared__ int32_t ring[RING ? 3 * RING * 64 : 1];
+    __shared__ uint8_t res[RING ? 2 * 32 * 64 : 1];
     for (int i = threadIdx.x; i < 32 * 32; i += blockDim.x)
         mat[i] = a.mat[i];
     __syncthreads();
@@ -236,22 +298,31 @@ __global__ __launch_bounds__(RING ? 64 : 256) void k_xdrop_pl(XdArgs a)
                                        (size_t)(3 * a.cap);
         A = Rings{buf, a.cap, 1, a.cap - 1, a.cap};
     }
+    const ResRings Rr{res + lane, res + 32 * 64 + lane};
+    constexpr bool ST = RING > 0;
     const int gap = a.gap, X = a.X;
     int state = kIdle;
     uint32_t k = 0;
     Item s{};
+    uint32_t cbase = 0, cend = 0;   // claimed, not yet handed out
     for (;;) {
-        // lanes without an item take the next ones: every lane joins the
-        // ballot, the atomic and the readfirstlane (lane 0 adds the count)
-        const uint64_t need = __ballot(state == kIdle);
-        if (need) {
+        // lanes without an item take the next ones from the wave's claimed
+        // range, which is refilled kChunk items per queue atomic; every lane
+        // runs this block (wave-uniform control flow)
+        uint64_t need = __ballot(state == kIdle);
+        while (need) {
+            if (cbase >= cend) {
+                uint32_t b = atomicAdd(a.queue, lane == 0 ? kChunk : 0u);
+                b = __builtin_amdgcn_readfirstlane(b);
+                cbase = b;
+                cend = b + kChunk;
+            }
             const uint32_t cnt = (uint32_t)__popcll(need);
-            uint32_t base = atomicAdd(a.queue, lane == 0 ? cnt : 0u);
-            base = __builtin_amdgcn_readfirstlane(base);
+            const uint32_t take = min(cnt, cend - cbase);
             const uint32_t rank =
                 (uint32_t)__popcll(need & ((1ull << lane) - 1ull));
-            if (state == kIdle) {
-                const uint32_t idx = base + rank;
+            if (((need >> lane) & 1ull) && rank < take) {
+                const uint32_t idx = cbase + rank;
                 if (idx >= n) {
                     state = kExhausted;
                 } else {
@@ -266,25 +337,39 @@ __global__ __launch_bounds__(RING ? 64 : 256) void k_xdrop_pl(XdArgs a)
                         s.cols = beg_v + 1;
                         s.ds = h;
                         s.rows = beg_h + 1;
+                        // reversed: column c is qs[cols-1-c], row r ds[rows-1-r]
+                        s.qb = s.qs + (s.cols - 1);
+                        s.qstep = -1;
+                        s.db = s.ds + (s.rows - 1);
+                        s.dstep = -1;
                     } else {          // V[end_v, nv), H[end_h, nh)
                         s.qs = v + beg_v + a.seed_len;
                         s.cols = (int)T.nv - beg_v - a.seed_len + 1;
                         s.ds = h + beg_h + a.seed_len;
                         s.rows = (int)T.nh - beg_h - a.seed_len + 1;
+                        s.qb = s.qs - 1;
+                        s.qstep = 1;
+                        s.db = s.ds - 1;
+                        s.dstep = 1;
                     }
                     if (s.rows == 1 || s.cols == 1) {
                         item_write(a, k, 0, 0, 0, false, 0, 0);
                     } else {
-                        item_init(s, A, gap, X);
+                        item_init<ST>(s, A, Rr, gap, X);
                         state = kRun;
                     }
                 }
             }
+            cbase += take;
+            // the lanes served leave `need`; a lane whose item was trivial
+            // asks again on the next trip
+            for (uint32_t t = 0; t < take; ++t)
+                need &= need - 1;
         }
         if (!__ballot(state != kExhausted))
             break;
         if (state == kRun) {
-            const int st = item_step(s, A, mat, gap, X);
+            const int st = item_step<ST>(s, A, Rr, mat, gap, X);
             if (st == 1) {
                 int32_t score;
                 int col, row;

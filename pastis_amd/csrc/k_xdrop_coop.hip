@@ -26,6 +26,7 @@ namespace {
 
 constexpr int32_t kUndef = INT_MIN / 4;
 constexpr int G = 16;   // lanes per item
+constexpr uint32_t kChunk = 32;   // items claimed per queue atomic
 
 // lane (j - k) mod 16 of the same row
 template <int K> __device__ inline int ror(int v)
@@ -99,19 +100,30 @@ __global__ __launch_bounds__(256) void k_xdrop_coop(CoopArgs a)
     uint32_t nc = 0;
     // this lane's values on anti-diagonals d-2, d-1, d
     int A1 = kUndef, A2 = kUndef, A3 = kUndef;
+    // items claimed by this wave and not yet handed out: [cbase, cend)
+    uint32_t cbase = 0, cend = 0;
 
     for (;;) {
-        // ---- hand out items to rows that need one (one atomic per wave)
+        // ---- hand out items to rows that need one.  The wave claims
+        // kChunk items per atomic and hands them out in row order; all
+        // lanes run this block together (wave-uniform control flow).
         {
-            const uint64_t need = __ballot(state == kNeed && j == 0);
-            if (need) {
+            uint64_t need = __ballot(state == kNeed && j == 0);
+            while (need) {
+                if (cbase >= cend) {
+                    uint32_t b = atomicAdd(a.queue, lane == 0 ? kChunk : 0u);
+                    b = __builtin_amdgcn_readfirstlane(b);
+                    cbase = b;
+                    cend = b + kChunk;
+                }
                 const uint32_t cnt = (uint32_t)__popcll(need);
-                uint32_t base = atomicAdd(a.queue, lane == 0 ? cnt : 0u);
-                base = __builtin_amdgcn_readfirstlane(base);
+                const uint32_t take = min(cnt, cend - cbase);
                 const uint32_t rank = (uint32_t)__popcll(
                     need & ((1ull << (16 * row)) - 1ull));
-                if (state == kNeed) {
-                    const uint32_t idx = base + rank;
+                const bool mine = ((need >> (16 * row)) & 1ull) != 0 &&
+                                  rank < take;
+                if (mine) {
+                    const uint32_t idx = cbase + rank;
                     if (idx >= n) {
                         state = kDone;
                     } else {
@@ -151,6 +163,9 @@ __global__ __launch_bounds__(256) void k_xdrop_coop(CoopArgs a)
                             state = 3;   // finished without an update
                     }
                 }
+                cbase += take;
+                for (uint32_t t = 0; t < take; ++t)
+                    need &= need - 1;   // the rows served
             }
         }
         if (!__ballot(state != kDone))

@@ -300,6 +300,8 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
             Scope sc(c, kind, s);
             for (int rcl = 0; rcl < 3; ++rcl) {
                 const int R = rcl == 0 ? 2 : (rcl == 1 ? 4 : 8);
+                // short tasks are claimed many per atomic, long ones singly
+                args.chunk = R == 2 ? 16u : (R == 4 ? 4u : 1u);
                 for (int w = 0; w < 2; ++w) {
                     const int bin = 2 * rcl + w;
                     args.list = lists + (size_t)bin * nt;

@@ -56,6 +56,7 @@ struct DpArgs {
     uint32_t *sat_list;        // tasks that saturated (int16 kernels)
     uint32_t *sat_count;
     int32_t gap_open, gap_ext;
+    uint32_t chunk;            // tasks claimed per queue atomic (>= 1)
 };
 
 // Arguments of the traceback kernel.
