@@ -80,12 +80,13 @@ template <> struct Ops<true> {
         uint32_t x = __builtin_amdgcn_perm(hw, lw, sel);
         return v(x) + splat(-128);
     }
-    // new row-above input: lo <- hi of lane-1, hi <- own lo
-    __device__ static inline V shuffle_up(V out)
+    // new row-above input: lo <- hi of lane-1, hi <- own lo (CTRL: DPP
+    // wave_shr:1 for 64-lane tasks, row_shr:1 for 16-lane tasks)
+    template <int CTRL> __device__ static inline V shuffle_up(V out)
     {
         uint32_t o = u(out);
         uint32_t sh = (uint32_t)__builtin_amdgcn_update_dpp(
-            0, (int)o, 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
+            0, (int)o, CTRL, 0xf, 0xf, false);
         return v(__builtin_amdgcn_alignbit(o, sh, 16));
     }
     __device__ static inline V set_lo(V a, int x)
@@ -143,9 +144,9 @@ template <> struct Ops<false> {
         return I2{(int)((lw >> (8 * K)) & 0xffu) - 128,
                   (int)((hw >> (8 * K)) & 0xffu) - 128};
     }
-    __device__ static inline V shuffle_up(V out)
+    template <int CTRL> __device__ static inline V shuffle_up(V out)
     {
-        int sh = __builtin_amdgcn_update_dpp(0, out.y, 0x138, 0xf, 0xf, false);
+        int sh = __builtin_amdgcn_update_dpp(0, out.y, CTRL, 0xf, 0xf, false);
         return I2{sh, out.x};
     }
     __device__ static inline V set_lo(V a, int x)
@@ -205,6 +206,24 @@ template <> struct TraceWord<8> {
     }
 };
 
+// One trace record (R bytes) to global memory.
+__device__ inline void store_word(uint8_t PA_GLOBAL *p, uint16_t w)
+{
+    *(uint16_t PA_GLOBAL *)p = w;
+}
+__device__ inline void store_word(uint8_t PA_GLOBAL *p, uint32_t w)
+{
+    *(uint32_t PA_GLOBAL *)p = w;
+}
+__device__ inline void store_word(uint8_t PA_GLOBAL *p, uint2 w)
+{
+    typedef uint32_t v2u __attribute__((ext_vector_type(2)));
+    v2u x;
+    x.x = w.x;
+    x.y = w.y;
+    *(v2u PA_GLOBAL *)p = x;
+}
+
 template <int R> struct ProfWords;
 template <> struct ProfWords<2> {
     __device__ static inline void load(const uint8_t *p, uint32_t *w)
@@ -245,9 +264,10 @@ struct ProfUnpack<O, R, 0> {
     }
 };
 
-__device__ inline uint64_t wave_max_u64(uint64_t v)
+// maximum over the L lanes of a task (L = 64: the wave)
+template <int L> __device__ inline uint64_t group_max_u64(uint64_t v)
 {
-    for (int off = 32; off >= 1; off >>= 1) {
+    for (int off = L / 2; off >= 1; off >>= 1) {
         uint32_t lo = __shfl_xor((uint32_t)v, off);
         uint32_t hi = __shfl_xor((uint32_t)(v >> 32), off);
         uint64_t o = ((uint64_t)hi << 32) | lo;
@@ -256,21 +276,30 @@ __device__ inline uint64_t wave_max_u64(uint64_t v)
     return v;
 }
 
-template <int R, bool LOCAL, bool TRACE, bool I16>
+// L lanes per task: 64 (one task per wave) or 16 (four tasks per wave, each
+// on one 16-lane DPP row; single-pass tasks only).
+template <int R, int L, bool LOCAL, bool TRACE, bool I16>
 __global__ __launch_bounds__(64) void k_dp(DpArgs a)
 {
     typedef Ops<I16> O;
     typedef typename O::V V;
     typedef typename O::Key Key;
-    constexpr int ROWS = 128 * R;
+    static_assert(L == 64 || L == 16, "lanes per task");
+    constexpr int G = 64 / L;                 // tasks per wave
+    constexpr int ROWS = 2 * R * L;           // rows per pass
+    constexpr int SHR = L == 64 ? 0x138 : 0x111;   // wave_shr:1 / row_shr:1
     constexpr int SH = TRACE ? 2 : 0;
     constexpr int NEG = TRACE ? O::kNegTraced : O::kNeg;
     HIP_DYNAMIC_SHARED(uint32_t, prof32)
-    const uint8_t *prof = (const uint8_t *)prof32;
-    // biased score matrix (score or 4*score+2, plus 128) after the profile;
-    // columns >= 25 and row 31 (padding rows) hold 0, i.e. -128
-    uint32_t *mb32 = prof32 + kProfChars * ROWS / 4;
     const int lane = threadIdx.x;
+    const int gl = lane % L;     // lane within the task
+    const int grp = lane / L;    // task slot of the wave
+    // query profile of this lane's task
+    uint32_t *gprof32 = prof32 + grp * (kProfChars * ROWS / 4);
+    const uint8_t *prof = (const uint8_t *)gprof32;
+    // biased score matrix (score or 4*score+2, plus 128) after the profiles;
+    // columns >= 25 and row 31 (padding rows) hold 0, i.e. -128
+    uint32_t *mb32 = prof32 + G * kProfChars * ROWS / 4;
     for (int w = lane; w < 256; w += 64) {
         uint32_t x = 0;
 #pragma unroll
@@ -292,7 +321,7 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
     const int gmax = open > ext ? open : ext;
     const V OPEN = O::splat(open);
     const V EXT = O::splat(TRACE ? ext + 1 : ext);
-    int32_t *bnd = a.bnd + (size_t)blockIdx.x * a.bnd_stride;
+    int32_t PA_GLOBAL *bnd = glb(a.bnd) + (size_t)blockIdx.x * a.bnd_stride;
 
     // tasks claimed by this wave and not yet started: [cbase, cend)
     uint32_t cbase = 0, cend = 0;
@@ -303,23 +332,36 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
         // readfirstlane lets the compiler rotate the loop so that the other
         // lanes re-enter it without lane 0.
         if (cbase >= cend) {
-            uint32_t b = atomicAdd(a.queue, lane == 0 ? chunk : 0u);
+            uint32_t b = atomicAdd(a.queue, lane == 0 ? chunk * G : 0u);
             b = __builtin_amdgcn_readfirstlane(b);
             cbase = b;
-            cend = b + chunk;
+            cend = b + chunk * G;
         }
-        const uint32_t k = cbase++;
-        if (k >= count)
+        const uint32_t k0 = cbase;
+        cbase += G;
+        if (k0 >= count)
             break;
-        const uint32_t tid = a.list[k];
-        const DpTask T = a.tasks[tid];
-        const uint8_t *qs = (const uint8_t *)(uintptr_t)T.q_off;
-        const uint8_t *ts = (const uint8_t *)(uintptr_t)T.t_off;
+        // task slot grp takes task k0 + grp (none past the end)
+        const uint32_t k = k0 + (uint32_t)grp;
+        const bool valid = k < count;
+        const uint32_t tid = glb(a.list)[valid ? k : k0];
+        DpTask T = a.tasks[tid];
+        if (!valid)
+            T.qlen = T.tlen = 0;
+        const uint8_t PA_GLOBAL *qs = glb_addr<const uint8_t>(T.q_off);
+        const uint8_t PA_GLOBAL *ts = glb_addr<const uint8_t>(T.t_off);
         const int qlen = (int)T.qlen, tlen = (int)T.tlen;
-        const int npass = (qlen + ROWS - 1) / ROWS;
-        uint8_t *tr = nullptr;
+        const int npass_own = (qlen + ROWS - 1) / ROWS;
+        int npass = npass_own;   // wave-uniform: the most passes of the slots
+        if (G > 1) {
+#pragma unroll
+            for (int q = 1; q < G; ++q)
+                npass = max(npass, __shfl(npass_own, q * L));
+            npass = max(npass, __shfl(npass_own, 0));
+        }
+        uint8_t PA_GLOBAL *tr = nullptr;
         if (TRACE)
-            tr = a.trace + a.trace_off[tid];
+            tr = glb(a.trace) + glb(a.trace_off)[tid];
 
         Key bk_lo = O::key(0, 0), bk_hi = O::key(0, 0);
         int bp_lo = 0, bp_hi = 0;
@@ -337,7 +379,7 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
             // gathering bytes of four matrix rows with v_perm
 #pragma unroll
             for (int g = 0; g < R / 2; ++g) {
-                const int row0 = i0 + 2 * R * lane + 4 * g;
+                const int row0 = i0 + 2 * R * gl + 4 * g;
                 int qc[4];
 #pragma unroll
                 for (int k = 0; k < 4; ++k)
@@ -351,7 +393,7 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
                     mr[k][3] = x0.w; mr[k][4] = x1.x; mr[k][5] = x1.y;
                     mr[k][6] = x1.z; mr[k][7] = x1.w;
                 }
-                uint32_t *dst = prof32 + (2 * R * lane + 4 * g) / 4;
+                uint32_t *dst = gprof32 + (2 * R * gl + 4 * g) / 4;
 #pragma unroll
                 for (int c = 0; c < kProfChars; ++c) {
                     const int wd = c >> 2, sh = c & 3;
@@ -369,10 +411,10 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
                 __threadfence();
             __syncthreads();
 
-            const int rlo = i0 + 2 * R * lane;
+            const int rlo = i0 + 2 * R * gl;
             const bool active = rlo < qlen;
             int nact = (qlen - i0 + 2 * R - 1) / (2 * R);
-            nact = nact > 64 ? 64 : nact;
+            nact = nact > L ? L : (nact < 0 ? 0 : nact);
             V H[R], E[R], Ho[R];
 #pragma unroll
             for (int r = 0; r < R; ++r) {
@@ -396,13 +438,21 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
                 bh1 = bnd[2];
                 bf1 = bnd[3];
             }
-            V upH = O::shuffle_up(outH), upF = O::shuffle_up(outF);
+            V upH = O::template shuffle_up<SHR>(outH);
+            V upF = O::template shuffle_up<SHR>(outF);
             V dg = O::splat(NEG);
-            upH = lane == 0 ? O::set_lo(upH, bh1) : upH;
-            upF = lane == 0 ? O::set_lo(upF, bf1) : upF;
-            dg = lane == 0 ? O::set_lo(dg, bh0) : dg;
+            upH = gl == 0 ? O::set_lo(upH, bh1) : upH;
+            upF = gl == 0 ? O::set_lo(upF, bf1) : upF;
+            dg = gl == 0 ? O::set_lo(dg, bh0) : dg;
             int c_lo = kSentinel, c_hi = kSentinel;
-            const int last_step = tlen + 2 * (nact - 1) + 1;
+            const int own_last = nact > 0 ? tlen + 2 * (nact - 1) + 1 : 0;
+            int last_step = own_last;   // wave-uniform: longest slot
+            if (G > 1) {
+#pragma unroll
+                for (int q = 1; q < G; ++q)
+                    last_step = max(last_step, __shfl(own_last, q * L));
+                last_step = max(last_step, __shfl(own_last, 0));
+            }
             // cell (qlen, tlen) for the global score
             const int qrow = qlen - 1 - i0;
             const int cap_lane = qrow / (2 * R);
@@ -412,9 +462,10 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
             const bool last_pass = p == npass - 1;
 
             // trace record of (step 1, this lane); one step adds rstep records
-            uint64_t rec = TRACE ? trace_rec(qlen, tlen, R, p, 1, lane) : 0;
-            const uint32_t rstep = TRACE ? (p + 1 < npass ? 64u : (uint32_t)nact)
-                                         : 0u;
+            uint64_t rec = TRACE && active ? trace_rec(qlen, tlen, R, p, 1, gl)
+                                           : 0;
+            const uint32_t rstep =
+                TRACE ? (p + 1 < npass_own ? (uint32_t)L : (uint32_t)nact) : 0u;
             // Column residues and (in later passes) the boundary row reach
             // lane 0 through v_readlane from registers filled 64 columns at a
             // time: lane l holds column base + l, and the next block is
@@ -424,11 +475,20 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
             auto res_at = [&](int j) {   // residue of column j (1-based)
                 return j <= tlen ? (int)ts[j - 1] : kSentinel;
             };
-            int rc_cur = res_at(1 + lane), rc_nxt = res_at(65 + lane);
+            // 64-lane tasks: lane l holds column base + l; 16-lane tasks:
+            // lane l of a slot holds columns base + 4l .. base + 4l + 3
+            auto res_blk = [&](int base) {   // block of 64 columns from base
+                if (L == 64)
+                    return res_at(base + lane);
+                const int c = base + 4 * gl;
+                return res_at(c) | (res_at(c + 1) << 8) |
+                       (res_at(c + 2) << 16) | (res_at(c + 3) << 24);
+            };
+            int rc_cur = res_blk(1), rc_nxt = res_blk(65);
             int bh_cur = 0, bf_cur = NEG, bh_nxt = 0, bf_nxt = NEG;
             if (p > 0) {
                 // boundary block k holds columns 64k .. 64k+63
-                if (lane <= tlen) {
+                if (lane <= tlen) {   // 64-lane tasks only
                     bh_cur = bnd[2 * lane];
                     bf_cur = bnd[2 * lane + 1];
                 }
@@ -445,13 +505,19 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
                     const int ri = (t - 1) & 63;
                     if (ri == 0 && t > 1) {
                         rc_cur = rc_nxt;
-                        rc_nxt = res_at(t + 64 + lane);
+                        rc_nxt = res_blk(t + 64);
                     }
                     const int nc = __builtin_amdgcn_update_dpp(
-                        0, c_hi, 0x138, 0xf, 0xf, false);
-                    const int c0 = __builtin_amdgcn_readlane(rc_cur, ri);
+                        0, c_hi, SHR, 0xf, 0xf, false);
+                    int c0;
+                    if (L == 64) {
+                        c0 = __builtin_amdgcn_readlane(rc_cur, ri);
+                    } else {
+                        const int w = __shfl(rc_cur, grp * L + (ri >> 2));
+                        c0 = (w >> (8 * (ri & 3))) & 0xff;
+                    }
                     c_hi = c_lo;
-                    c_lo = lane == 0 ? c0 : nc;
+                    c_lo = gl == 0 ? c0 : nc;
                 }
                 int nbh = 0, nbf = NEG;  // boundary for lane 0 at step t+1
                 {
@@ -473,12 +539,12 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
                         }
                     }
                 }
-                const int jlo = t - 2 * lane;
+                const int jlo = t - 2 * gl;
                 const int jhi = jlo - 1;
                 if (active && jlo >= 1 && jlo <= tlen + 1) {
                     uint32_t lw[(R + 3) / 4], hw[(R + 3) / 4];
-                    ProfWords<R>::load(prof + c_lo * ROWS + 2 * R * lane, lw);
-                    ProfWords<R>::load(prof + c_hi * ROWS + 2 * R * lane + R,
+                    ProfWords<R>::load(prof + c_lo * ROWS + 2 * R * gl, lw);
+                    ProfWords<R>::load(prof + c_hi * ROWS + 2 * R * gl + R,
                                        hw);
                     V s[R];
                     ProfUnpack<O, R, R>::run(lw, hw, s);
@@ -531,7 +597,7 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
                     if (TRACE) {
                         typedef typename TraceWord<R>::T W;
                         W w = TraceWord<R>::pack(codes);
-                        *(W *)(tr + rec * R) = w;
+                        store_word(tr + rec * R, w);
                     }
                     if (LOCAL) {
                         const bool vlo = jlo <= tlen;
@@ -548,22 +614,22 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
                             for (int r = 0; r < R; ++r)
                                 saved[r] = O::blend(saved[r], H[r], ilo, ihi);
                         }
-                    } else if (last_pass && lane == cap_lane && t == cap_step) {
+                    } else if (last_pass && gl == cap_lane && t == cap_step) {
 #pragma unroll
                         for (int r = 0; r < R; ++r)
                             if (r == cap_r)
                                 gscore = cap_half ? O::hi(H[r]) : O::lo(H[r]);
                     }
-                    if (p + 1 < npass && lane == 63 && jhi >= 0 && jhi <= tlen) {
+                    if (p + 1 < npass && gl == L - 1 && jhi >= 0 && jhi <= tlen) {
                         bnd[2 * jhi] = O::hi(outH);
                         bnd[2 * jhi + 1] = O::hi(outF);
                     }
                 }
                 dg = upH;
-                upH = O::shuffle_up(outH);
-                upF = O::shuffle_up(outF);
-                upH = lane == 0 ? O::set_lo(upH, nbh) : upH;
-                upF = lane == 0 ? O::set_lo(upF, nbf) : upF;
+                upH = O::template shuffle_up<SHR>(outH);
+                upF = O::template shuffle_up<SHR>(outF);
+                upH = gl == 0 ? O::set_lo(upH, nbh) : upH;
+                upF = gl == 0 ? O::set_lo(upF, nbf) : upF;
             }
         }
 
@@ -579,7 +645,7 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
                 if (sc <= 0)
                     continue;
                 const int col = O::key_col(bk);
-                const int rbase = bp * ROWS + 2 * R * lane + hf * R;
+                const int rbase = bp * ROWS + 2 * R * gl + hf * R;
                 int row = -1;
 #pragma unroll
                 for (int r = R - 1; r >= 0; --r) {
@@ -594,7 +660,7 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
                                     (uint64_t)(0xffffff - row);
                 best = ck > best ? ck : best;
             }
-            best = wave_max_u64(best);
+            best = group_max_u64<L>(best);
             const int sc = (int)(best >> 40);
             if (sc > 0) {
                 o.score = sc;
@@ -602,11 +668,11 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
                 o.q_end = 0xffffff - (int)(best & 0xffffff) + 1;
             }
             if (I16) {
+                // a saturated task is appended by its slot's first lane (no
+                // wave operation depends on the atomic)
                 const int lim = TRACE ? 8190 : 32766;
-                if (sc >= lim) {  // wave-uniform
-                    uint32_t pos =
-                        atomicAdd(a.sat_count, lane == 0 ? 1u : 0u);
-                    pos = __builtin_amdgcn_readfirstlane(pos);
+                if (valid && sc >= lim && gl == 0) {
+                    const uint32_t pos = atomicAdd(a.sat_count, 1u);
                     a.sat_list[pos] = tid;
                 }
             }
@@ -614,16 +680,17 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
             // the capturing lane of the last pass holds H(qlen, tlen)
             const int qrow = qlen - 1 - (npass - 1) * ROWS;
             const int cl = qrow / (2 * R);
-            int g = __shfl(gscore, cl);
+            int g = __shfl(gscore, grp * L + (cl & (L - 1)));
             o.score = g >> SH;
             o.q_end = qlen;
             o.t_end = tlen;
         }
-        // all lanes hold the same result; every lane stores it (same
-        // address, same bytes) so no lane-0 branch closes the task loop
+        // all lanes of a slot hold the same result; every lane stores it
+        // (same address, same bytes) so no lane-0 branch closes the loop
         o.q_beg = o.q_end;
         o.t_beg = o.t_end;
-        a.out[tid] = o;
+        if (valid)
+            a.out[tid] = o;
     }
 }
 
@@ -639,7 +706,7 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
 namespace pa {
 namespace {
 
-__device__ inline int trace_code(const uint8_t *tr, int R, int qlen,
+__device__ inline int trace_code(const uint8_t PA_GLOBAL *tr, int R, int qlen,
                                  int tlen, int i, int j)
 {
     const int rows = 128 * R;
@@ -659,14 +726,14 @@ __global__ __launch_bounds__(256) void k_traceback(TbArgs a)
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= *a.count)
         return;
-    const uint32_t tid = a.list[k];
+    const uint32_t tid = glb(a.list)[k];
     const DpTask T = a.tasks[tid];
     DpOut o = a.out[tid];
-    const uint8_t *qs = (const uint8_t *)(uintptr_t)T.q_off;
-    const uint8_t *ts = (const uint8_t *)(uintptr_t)T.t_off;
+    const uint8_t PA_GLOBAL *qs = glb_addr<const uint8_t>(T.q_off);
+    const uint8_t PA_GLOBAL *ts = glb_addr<const uint8_t>(T.t_off);
     const int tlen = (int)T.tlen;
     const int R = rows_per_lane(T.qlen);
-    const uint8_t *tr = a.trace + a.trace_off[tid];
+    const uint8_t PA_GLOBAL *tr = glb(a.trace) + glb(a.trace_off)[tid];
     int i = o.q_end, j = o.t_end;
     int matches = 0, cols = 0, runs = 0;
     int last = 0;   // 1 diagonal, 2 vertical, 3 horizontal
@@ -727,76 +794,81 @@ __global__ __launch_bounds__(256) void k_traceback(TbArgs a)
     a.out[tid] = o;
 }
 
-template <int R, bool L, bool T, bool I>
+template <int R, int L, bool LC, bool T, bool I>
 hipError_t launch_one(const DpArgs &a, int grid, hipStream_t s)
 {
+    // G task profiles of 2RL rows each (128R rows in all) + the matrix
     const size_t lds = (size_t)kProfChars * 128 * R + 1024;
-    hipLaunchKernelGGL((k_dp<R, L, T, I>), dim3(grid), dim3(64), lds, s, a);
+    hipLaunchKernelGGL((k_dp<R, L, LC, T, I>), dim3(grid), dim3(64), lds, s,
+                       a);
     return hipGetLastError();
 }
 
-template <int R, bool L, bool T, bool I>
+template <int R, int L, bool LC, bool T, bool I>
 int occupancy_one()
 {
     const size_t lds = (size_t)kProfChars * 128 * R + 1024;
     int n = 0;
-    hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_dp<R, L, T, I>, 64,
+    hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_dp<R, L, LC, T, I>, 64,
                                                  lds);
     return n > 0 ? n : 1;
 }
 
-template <int R>
-int occupancy_r(bool local, bool trace, bool i16)
+template <int R, int L>
+int occupancy_c(bool local, bool trace, bool i16)
 {
     if (local) {
         if (trace)
-            return i16 ? occupancy_one<R, true, true, true>()
-                       : occupancy_one<R, true, true, false>();
-        return i16 ? occupancy_one<R, true, false, true>()
-                   : occupancy_one<R, true, false, false>();
+            return i16 ? occupancy_one<R, L, true, true, true>()
+                       : occupancy_one<R, L, true, true, false>();
+        return i16 ? occupancy_one<R, L, true, false, true>()
+                   : occupancy_one<R, L, true, false, false>();
     }
-    return i16 ? occupancy_one<R, false, true, true>()
-               : occupancy_one<R, false, true, false>();
+    return i16 ? occupancy_one<R, L, false, true, true>()
+               : occupancy_one<R, L, false, true, false>();
 }
 
-template <int R>
-hipError_t launch_r(bool local, bool trace, bool i16, const DpArgs &a,
+template <int R, int L>
+hipError_t launch_c(bool local, bool trace, bool i16, const DpArgs &a,
                     int grid, hipStream_t s)
 {
     if (local) {
         if (trace)
-            return i16 ? launch_one<R, true, true, true>(a, grid, s)
-                       : launch_one<R, true, true, false>(a, grid, s);
-        return i16 ? launch_one<R, true, false, true>(a, grid, s)
-                   : launch_one<R, true, false, false>(a, grid, s);
+            return i16 ? launch_one<R, L, true, true, true>(a, grid, s)
+                       : launch_one<R, L, true, true, false>(a, grid, s);
+        return i16 ? launch_one<R, L, true, false, true>(a, grid, s)
+                   : launch_one<R, L, true, false, false>(a, grid, s);
     }
-    return i16 ? launch_one<R, false, true, true>(a, grid, s)
-               : launch_one<R, false, true, false>(a, grid, s);
+    return i16 ? launch_one<R, L, false, true, true>(a, grid, s)
+               : launch_one<R, L, false, true, false>(a, grid, s);
 }
 
 }  // namespace
 
-hipError_t launch_dp(int R, bool local, bool trace, bool i16, const DpArgs &a,
-                     int grid, hipStream_t s)
+hipError_t launch_dp(int cls, bool local, bool trace, bool i16,
+                     const DpArgs &a, int grid, hipStream_t s)
 {
-    switch (R) {
-    case 2: return launch_r<2>(local, trace, i16, a, grid, s);
-    case 4: return launch_r<4>(local, trace, i16, a, grid, s);
-    case 8: return launch_r<8>(local, trace, i16, a, grid, s);
+    switch (cls) {
+    case 0: return launch_c<4, 16>(local, trace, i16, a, grid, s);
+    case 1: return launch_c<8, 16>(local, trace, i16, a, grid, s);
+    case 2: return launch_c<4, 64>(local, trace, i16, a, grid, s);
+    case 3: return launch_c<8, 64>(local, trace, i16, a, grid, s);
     default: return hipErrorInvalidValue;
     }
 }
 
-int dp_occupancy(int R, bool local, bool trace, bool i16)
+int dp_occupancy(int cls, bool local, bool trace, bool i16)
 {
-    static int cache[3][8] = {};
-    const int ri = R == 2 ? 0 : (R == 4 ? 1 : 2);
+    static int cache[4][8] = {};
     const int vi = (local ? 4 : 0) + (trace ? 2 : 0) + (i16 ? 1 : 0);
-    if (!cache[ri][vi])
-        cache[ri][vi] = R == 2   ? occupancy_r<2>(local, trace, i16)
-                        : R == 4 ? occupancy_r<4>(local, trace, i16)
-                                 : occupancy_r<8>(local, trace, i16);
-    return cache[ri][vi];
+    if (cls < 0 || cls > 3)
+        return 1;
+    if (!cache[cls][vi])
+        cache[cls][vi] = cls == 0   ? occupancy_c<4, 16>(local, trace, i16)
+                         : cls == 1 ? occupancy_c<8, 16>(local, trace, i16)
+                         : cls == 2 ? occupancy_c<4, 64>(local, trace, i16)
+                                    : occupancy_c<8, 64>(local, trace, i16);
+    return cache[cls][vi];
 }
 
 hipError_t launch_traceback(const TbArgs &a, int grid, hipStream_t s)

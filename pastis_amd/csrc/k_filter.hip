@@ -249,21 +249,27 @@ __global__ __launch_bounds__(256) void k_logits_of(
     logit_out[i] = mlp_logit(m.w, xb);
 }
 
-// keep pairs whose logit >= tau_logit; order of the kept list is arbitrary
-__global__ void k_select(const pa_pair *__restrict__ pairs,
-                         const float *__restrict__ logits, uint64_t n,
-                         float tau, pa_pair *__restrict__ kept,
-                         uint32_t *__restrict__ kept_index, uint32_t *count)
+// keep pairs whose logit >= tau_logit; order of the kept list is arbitrary.
+// The block counts its kept pairs in LDS and reserves one range with a
+// single global atomic.
+__global__ __launch_bounds__(256) void k_select(
+    const pa_pair *__restrict__ pairs, const float *__restrict__ logits,
+    uint64_t n, float tau, pa_pair *__restrict__ kept,
+    uint32_t *__restrict__ kept_index, uint32_t *count)
 {
+    __shared__ uint32_t nk, base;
+    if (threadIdx.x == 0)
+        nk = 0;
+    __syncthreads();
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool keep = i < n && logits[i] >= tau;
-    const uint64_t m = __ballot(keep);
-    const int lane = threadIdx.x & 63;
-    const int leader = m ? __ffsll((unsigned long long)m) - 1 : 0;
-    uint32_t base = atomicAdd(count, lane == leader ? (uint32_t)__popcll(m) : 0u);
-    base = __shfl(base, leader);
+    const uint32_t rank = keep ? atomicAdd(&nk, 1u) : 0u;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        base = nk ? atomicAdd(count, nk) : 0u;
+    __syncthreads();
     if (keep) {
-        const uint32_t pos = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        const uint32_t pos = base + rank;
         kept[pos] = pairs[i];
         if (kept_index)
             kept_index[pos] = (uint32_t)i;

@@ -5,8 +5,12 @@
 
 namespace pa {
 
-// Bins: 2 * R-class (R = 2, 4, 8) + (1 for the int32 kernels).
-constexpr int kBins = 6;
+// Bins: 2 * DP class (dp_class) + (1 for the int32 kernels).  Inside a bin
+// the tasks of the 16-lane classes are ordered by kBuckets buckets of their
+// step count, so that the four tasks a wave runs together are alike.
+constexpr int kClasses = 4;
+constexpr int kBins = 2 * kClasses;
+constexpr int kBuckets = 16;
 
 hipError_t fill_u32(uint32_t *p, uint32_t v, uint32_t n, hipStream_t s);
 hipError_t build_sw_tasks(const pa_pair *pairs, uint64_t n,
@@ -19,10 +23,14 @@ hipError_t build_xd_tasks(const pa_pair *pairs, uint64_t n, int seed_cnt,
 hipError_t build_nw_tasks(const XdTask *xt, uint64_t nt, const XdOut *xo,
                           DpTask *tasks, unsigned long long *cells,
                           unsigned long long *xcells, hipStream_t s);
+// Counting sort of tasks [a, b) into per-bin lists: hist and fill are
+// kBins * kBuckets counters that start at zero, counts receives the kBins
+// list lengths.
 hipError_t bin_tasks(const DpTask *tasks, uint32_t a, uint32_t b, int local,
                      int maxs, int open, int ext, uint32_t *lists,
-                     uint32_t cap, uint32_t *counts,
-                     unsigned long long *bin_cells, hipStream_t s);
+                     uint32_t cap, uint32_t *counts, uint32_t *hist,
+                     uint32_t *fill, unsigned long long *bin_cells,
+                     hipStream_t s);
 hipError_t trace_sizes(const DpTask *tasks, uint64_t n, uint64_t *sizes,
                        hipStream_t s);
 hipError_t empty_out(const DpTask *tasks, uint32_t a, uint32_t b, int local,

@@ -114,47 +114,99 @@ __device__ inline bool global_i16_ok(uint32_t q, uint32_t t, int maxs,
     return 4 * ub + 3 < 32760 && 4 * lb > -31000;
 }
 
-// Tasks [a, b) -> per-bin lists.  Bin = 2*R-class + (int32 ? 1 : 0).
-__global__ void k_bin(const DpTask *__restrict__ tasks, uint32_t a, uint32_t b,
-                      int local, int maxs, int open, int ext,
-                      uint32_t *__restrict__ lists, uint32_t cap,
-                      uint32_t *counts, unsigned long long *bin_cells)
+// Bin and bucket of a task (-1: empty task).  The bucket of a 16-lane class
+// task is its step count / 32 (capped); other classes use bucket 0.
+__device__ inline int bin_key(const DpTask &t, int local, int maxs, int open,
+                              int ext)
 {
+    if (t.qlen == 0 || t.tlen == 0)
+        return -1;
+    const int rc = dp_class(t.qlen);
+    const bool i16 =
+        local ? true : global_i16_ok(t.qlen, t.tlen, maxs, open, ext);
+    const int bin = 2 * rc + (i16 ? 0 : 1);
+    int bucket = 0;
+    if (rc <= 1) {
+        const int R = rows_per_lane(t.qlen);
+        const int nact = ((int)t.qlen + 2 * R - 1) / (2 * R);
+        const int steps = (int)t.tlen + 2 * (nact - 1) + 1;
+        bucket = min(kBuckets - 1, steps >> 5);
+    }
+    return bin * kBuckets + bucket;
+}
+
+// Pass 1: histogram of the (bin, bucket) keys: LDS counters per block, then
+// one global atomic per key that occurs in the block.
+__global__ __launch_bounds__(256) void k_bin_hist(
+    const DpTask *__restrict__ tasks, uint32_t a, uint32_t b, int local,
+    int maxs, int open, int ext, uint32_t *hist)
+{
+    __shared__ uint32_t h[kBins * kBuckets];
+    for (int k = threadIdx.x; k < kBins * kBuckets; k += blockDim.x)
+        h[k] = 0;
+    __syncthreads();
     const uint32_t i = a + blockIdx.x * blockDim.x + threadIdx.x;
-    int bin = -1;
+    const int key = i < b ? bin_key(tasks[i], local, maxs, open, ext) : -1;
+    if (key >= 0)
+        atomicAdd(&h[key], 1u);
+    __syncthreads();
+    for (int k = threadIdx.x; k < kBins * kBuckets; k += blockDim.x)
+        if (h[k])
+            atomicAdd(&hist[k], h[k]);
+}
+
+// Pass 2 (one block of 64 threads): fill = exclusive scan of the buckets of
+// each bin, counts = bin totals.
+__global__ void k_bin_scan(const uint32_t *hist, uint32_t *fill,
+                           uint32_t *counts)
+{
+    const int bin = threadIdx.x;
+    if (bin >= kBins)
+        return;
+    uint32_t run = 0;
+    for (int k = 0; k < kBuckets; ++k) {
+        fill[bin * kBuckets + k] = run;
+        run += hist[bin * kBuckets + k];
+    }
+    counts[bin] = run;
+}
+
+// Pass 3: each task goes to its bin's list at its bucket's next slot.  The
+// block counts its keys in LDS (the LDS atomic's old value is the task's
+// rank), reserves one range per key with a global atomic, and scatters.
+__global__ __launch_bounds__(256) void k_bin(
+    const DpTask *__restrict__ tasks, uint32_t a, uint32_t b, int local,
+    int maxs, int open, int ext, uint32_t *__restrict__ lists, uint32_t cap,
+    uint32_t *fill, unsigned long long *bin_cells)
+{
+    __shared__ uint32_t h[kBins * kBuckets], base[kBins * kBuckets];
+    __shared__ unsigned long long cc[kBins];
+    for (int k = threadIdx.x; k < kBins * kBuckets; k += blockDim.x)
+        h[k] = 0;
+    if (threadIdx.x < kBins)
+        cc[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t i = a + blockIdx.x * blockDim.x + threadIdx.x;
+    int key = -1;
     uint64_t cells = 0;
     if (i < b) {
         const DpTask t = tasks[i];
+        key = bin_key(t, local, maxs, open, ext);
         cells = (uint64_t)t.qlen * t.tlen;
-        if (t.qlen > 0 && t.tlen > 0) {
-            const int R = rows_per_lane(t.qlen);
-            const int rc = R == 2 ? 0 : (R == 4 ? 1 : 2);
-            const bool i16 =
-                local ? true : global_i16_ok(t.qlen, t.tlen, maxs, open, ext);
-            bin = 2 * rc + (i16 ? 0 : 1);
-        }
     }
-    const int lane = threadIdx.x & 63;
-    for (int bb = 0; bb < kBins; ++bb) {
-        const uint64_t m = __ballot(bin == bb);
-        if (!m)
-            continue;
-        const int leader = __ffsll((unsigned long long)m) - 1;
-        // every lane joins the atomic (only the leader adds) so that no
-        // divergent branch precedes the shuffle
-        uint32_t base = atomicAdd(&counts[bb],
-                                  lane == leader ? (uint32_t)__popcll(m) : 0u);
-        base = __shfl(base, leader);
-        if (bin == bb) {
-            const uint32_t rank =
-                __popcll(m & ((1ull << lane) - 1ull));
-            lists[(size_t)bb * cap + base + rank] = i;
-        }
-        uint64_t c = bin == bb ? cells : 0;
-        for (int off = 32; off >= 1; off >>= 1)
-            c += __shfl_xor(c, off);
-        atomicAdd(&bin_cells[bb], lane == leader ? (unsigned long long)c : 0ull);
+    uint32_t rank = 0;
+    if (key >= 0) {
+        rank = atomicAdd(&h[key], 1u);
+        atomicAdd(&cc[key / kBuckets], (unsigned long long)cells);
     }
+    __syncthreads();
+    for (int k = threadIdx.x; k < kBins * kBuckets; k += blockDim.x)
+        base[k] = h[k] ? atomicAdd(&fill[k], h[k]) : 0u;
+    if (threadIdx.x < kBins && cc[threadIdx.x])
+        atomicAdd(&bin_cells[threadIdx.x], cc[threadIdx.x]);
+    __syncthreads();
+    if (key >= 0)
+        lists[(size_t)(key / kBuckets) * cap + base[key] + rank] = i;
 }
 
 __global__ void k_trace_sizes(const DpTask *__restrict__ tasks, uint64_t n,
@@ -380,13 +432,18 @@ hipError_t build_nw_tasks(const XdTask *xt, uint64_t nt, const XdOut *xo,
 
 hipError_t bin_tasks(const DpTask *tasks, uint32_t a, uint32_t b, int local,
                      int maxs, int open, int ext, uint32_t *lists,
-                     uint32_t cap, uint32_t *counts,
-                     unsigned long long *bin_cells, hipStream_t s)
+                     uint32_t cap, uint32_t *counts, uint32_t *hist,
+                     uint32_t *fill, unsigned long long *bin_cells,
+                     hipStream_t s)
 {
     if (b <= a)
         return hipSuccess;
+    hipLaunchKernelGGL(k_bin_hist, dim3(blocks(b - a, 256)), dim3(256), 0, s,
+                       tasks, a, b, local, maxs, open, ext, hist);
+    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(64), 0, s, hist, fill,
+                       counts);
     hipLaunchKernelGGL(k_bin, dim3(blocks(b - a, 256)), dim3(256), 0, s, tasks,
-                       a, b, local, maxs, open, ext, lists, cap, counts,
+                       a, b, local, maxs, open, ext, lists, cap, fill,
                        bin_cells);
     return hipGetLastError();
 }

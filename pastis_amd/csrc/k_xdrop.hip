@@ -43,7 +43,7 @@ struct Rings {
 
 // Resumable state of one direction's extension.
 struct Item {
-    const uint8_t *qs, *ds;   // query (V, columns) and database (H, rows)
+    const uint8_t PA_GLOBAL *qs, *ds;   // query (V, columns), database (H, rows)
     int cols, rows;           // segment lengths + 1
     bool right;
     int d, minCol, maxCol, best;
@@ -52,8 +52,8 @@ struct Item {
     uint32_t nc;              // evaluated cells
     // residue staging (LDS kernels): the request issued at the start of the
     // previous anti-diagonal is stored at the start of this one
-    const uint8_t *qb, *db;   // residue of column c at qb[qstep * c], row r
-    int qstep, dstep;         // at db[dstep * r]
+    const uint8_t PA_GLOBAL *qb, *db;   // column c: qb[qstep * c], row r:
+    int qstep, dstep;                   // db[dstep * r]
     int qhi, rhi;             // highest column / row requested
     int rq, rcol, rd, rrow;   // request in flight: residues, column, row
 };
@@ -257,7 +257,7 @@ __device__ inline void item_write(const XdArgs &a, uint32_t k, int32_t score,
                                   uint16_t flags)
 {
     const XdTask T = a.tasks[k >> 1];
-    XdOut &o = a.out[T.id];
+    XdOut PA_GLOBAL &o = glb(a.out)[T.id];
     const int beg_h = (int)T.beg_h, beg_v = (int)T.beg_v;
     if ((k & 1) == 0) {
         o.beg_h = upd ? beg_h - row : beg_h;
@@ -326,10 +326,10 @@ __global__ __launch_bounds__(RING ? 64 : 256) void k_xdrop_pl(XdArgs a)
                 if (idx >= n) {
                     state = kExhausted;
                 } else {
-                    k = a.list ? a.list[idx] : idx;
+                    k = a.list ? glb(a.list)[idx] : idx;
                     const XdTask T = a.tasks[k >> 1];
-                    const uint8_t *h = (const uint8_t *)(uintptr_t)T.h_off;
-                    const uint8_t *v = (const uint8_t *)(uintptr_t)T.v_off;
+                    const uint8_t PA_GLOBAL *h = glb_addr<const uint8_t>(T.h_off);
+                    const uint8_t PA_GLOBAL *v = glb_addr<const uint8_t>(T.v_off);
                     const int beg_h = (int)T.beg_h, beg_v = (int)T.beg_v;
                     s.right = (k & 1) != 0;
                     if (!s.right) {   // query V[0, beg_v), database H[0, beg_h)

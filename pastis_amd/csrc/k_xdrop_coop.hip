@@ -93,7 +93,7 @@ __global__ __launch_bounds__(256) void k_xdrop_coop(CoopArgs a)
     // row-uniform state of the current item
     int state = kNeed;
     uint32_t k = 0;
-    const uint8_t *qs = nullptr, *ds = nullptr;
+    const uint8_t PA_GLOBAL *qs = nullptr, *ds = nullptr;
     int cols = 0, rows = 0, d = 0, minCol = 0, maxCol = 0, best = 0;
     int off1 = 0, off2 = 0, off3 = 0, l1 = 0, l2 = 0, l3 = 0;
     bool right = false;
@@ -127,10 +127,10 @@ __global__ __launch_bounds__(256) void k_xdrop_coop(CoopArgs a)
                     if (idx >= n) {
                         state = kDone;
                     } else {
-                        k = a.list ? a.list[idx] : idx;
+                        k = a.list ? glb(a.list)[idx] : idx;
                         const XdTask T = a.tasks[k >> 1];
-                        const uint8_t *h = (const uint8_t *)(uintptr_t)T.h_off;
-                        const uint8_t *v = (const uint8_t *)(uintptr_t)T.v_off;
+                        const uint8_t PA_GLOBAL *h = glb_addr<const uint8_t>(T.h_off);
+                        const uint8_t PA_GLOBAL *v = glb_addr<const uint8_t>(T.v_off);
                         const int beg_h = (int)T.beg_h, beg_v = (int)T.beg_v;
                         right = (k & 1) != 0;
                         if (!right) {   // query V[0, beg_v), database H[0, beg_h)
@@ -195,8 +195,10 @@ __global__ __launch_bounds__(256) void k_xdrop_coop(CoopArgs a)
             const int qp = right ? c - 1 : cols - 1 - c;
             const int dp = right ? nd - c - 1 : rows - 1 + c - nd;
             // rows without a live item read a byte of the score table
-            const uint8_t *qa = ld ? qs + qp : (const uint8_t *)a.mat;
-            const uint8_t *da = ld ? ds + dp : (const uint8_t *)a.mat;
+            const uint8_t PA_GLOBAL *qa =
+                ld ? qs + qp : glb((const uint8_t *)a.mat);
+            const uint8_t PA_GLOBAL *da =
+                ld ? ds + dp : glb((const uint8_t *)a.mat);
             const int sc = mat[(int)(*qa & 31) * 32 + (*da & 31)];
             int tmp = max(lA3, A3) + gap;
             tmp = max(tmp, lA2 + sc);
@@ -280,7 +282,7 @@ __global__ __launch_bounds__(256) void k_xdrop_coop(CoopArgs a)
             }
             if (fin && j == 0) {
                 const XdTask T = a.tasks[k >> 1];
-                XdOut &o = a.out[T.id];
+                XdOut PA_GLOBAL &o = glb(a.out)[T.id];
                 const bool upd = state != 3 && ls != kUndef;
                 const int beg_h = (int)T.beg_h, beg_v = (int)T.beg_v;
                 const uint16_t fl = state == 4 ? 1 : 0;

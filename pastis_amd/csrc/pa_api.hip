@@ -72,18 +72,20 @@ const int8_t kBlosum62[25 * 25] = {
 // Work-queue and counter slots (uint32) of the per-chunk counter block.
 enum {
     C_BIN = 0,          // kBins list lengths
-    C_BINQ = 6,         // kBins work-queue heads
-    C_SAT = 12,         // 3 saturation-list lengths (per R class)
-    C_SATQ = 15,        // 3 re-run queue heads
-    C_XDQ = 18,         // X-drop queue head
-    C_XDOVF = 19,       // X-drop overflow count
-    C_XDOVFQ = 20,      // X-drop overflow queue head
-    C_XDN = 21,         // X-drop task count
-    C_XDDUMMY = 22,     // overflow sink of the wide re-run
-    C_ERR = 23,         // validation errors
-    C_XDLONG = 24,      // X-drop items that outlived the first-pass budget
-    C_XDLONGQ = 25,     // their queue head
-    C_N = 32
+    C_BINQ = 8,         // kBins work-queue heads
+    C_SAT = 16,         // kClasses saturation-list lengths
+    C_SATQ = 20,        // kClasses re-run queue heads
+    C_XDQ = 24,         // X-drop queue head
+    C_XDOVF = 25,       // X-drop overflow count
+    C_XDOVFQ = 26,      // X-drop overflow queue head
+    C_XDN = 27,         // X-drop task count
+    C_XDDUMMY = 28,     // overflow sink of the wide re-run
+    C_ERR = 29,         // validation errors
+    C_XDLONG = 30,      // X-drop items left by the 16-lane kernel
+    C_XDLONGQ = 31,     // their queue head
+    C_HIST = 32,        // kBins x kBuckets histogram of the counting sort
+    C_FILL = C_HIST + kBins * kBuckets,   // its running slots
+    C_N = C_FILL + kBins * kBuckets
 };
 
 constexpr int kMaxWgPerCu = 32;     // one-wave workgroups per CU (8 per SIMD)
@@ -165,17 +167,17 @@ int ensure_common(pa_ctx *c, uint64_t nt)
         return rc;
     if ((rc = grow(c, c->lists, (size_t)kBins * nt * sizeof(uint32_t))))
         return rc;
-    if ((rc = grow(c, c->sat, 3 * nt * sizeof(uint32_t))))
+    if ((rc = grow(c, c->sat, (size_t)kClasses * nt * sizeof(uint32_t))))
         return rc;
     return PA_OK;
 }
 
 // Persistent DP grid: occupancy-limited workgroups per CU, capped by the
 // tuning and by the number of tasks.
-int dp_grid(pa_ctx *c, uint64_t ntasks, int R, bool local, bool trace,
+int dp_grid(pa_ctx *c, uint64_t ntasks, int cls, bool local, bool trace,
             bool i16)
 {
-    int per_cu = dp_occupancy(R, local, trace, i16);
+    int per_cu = dp_occupancy(cls, local, trace, i16);
     if (c->tune.dp_wg_per_cu > 0)
         per_cu = std::min(per_cu, (int)c->tune.dp_wg_per_cu);
     per_cu = std::min(per_cu, kMaxWgPerCu);
@@ -276,9 +278,11 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
         if (trace && chunks.size() > 1)
             base = c->h_toffs[a];
         HIPCHK(c, fill_u32(ctr, 0, C_XDQ, s));
+        HIPCHK(c, fill_u32(ctr + C_HIST, 0, C_N - C_HIST, s));
         HIPCHK(c, bin_tasks(P<DpTask>(c->tasks), a, b, local ? 1 : 0, c->maxs,
                             c->cfg.gap_open, c->cfg.gap_ext, lists,
-                            (uint32_t)nt, ctr + C_BIN,
+                            (uint32_t)nt, ctr + C_BIN, ctr + C_HIST,
+                            ctr + C_FILL,
                             (unsigned long long *)P<uint64_t>(c->bincells) +
                                 dmode * kBins,
                             s));
@@ -298,10 +302,10 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
         args.gap_ext = c->cfg.gap_ext;
         {
             Scope sc(c, kind, s);
-            for (int rcl = 0; rcl < 3; ++rcl) {
-                const int R = rcl == 0 ? 2 : (rcl == 1 ? 4 : 8);
+            for (int rcl = 0; rcl < kClasses; ++rcl) {
                 // short tasks are claimed many per atomic, long ones singly
-                args.chunk = R == 2 ? 16u : (R == 4 ? 4u : 1u);
+                // (the 16-lane classes claim rounds of four tasks)
+                args.chunk = rcl <= 2 ? 4u : 1u;
                 for (int w = 0; w < 2; ++w) {
                     const int bin = 2 * rcl + w;
                     args.list = lists + (size_t)bin * nt;
@@ -311,8 +315,8 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
                     args.sat_count = ctr + C_SAT + rcl;
                     {
                         Scope k(c, 100 + dmode * 16 + bin, s);
-                        HIPCHK(c, launch_dp(R, local, trace, w == 0, args,
-                                            dp_grid(c, b - a, R, local, trace,
+                        HIPCHK(c, launch_dp(rcl, local, trace, w == 0, args,
+                                            dp_grid(c, b - a, rcl, local, trace,
                                                     w == 0),
                                             s));
                     }
@@ -325,8 +329,8 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
                     args.queue = ctr + C_SATQ + rcl;
                     {
                         Scope k(c, 100 + dmode * 16 + kBins + rcl, s);
-                        HIPCHK(c, launch_dp(R, local, trace, false, args,
-                                            dp_grid(c, b - a, R, local, trace,
+                        HIPCHK(c, launch_dp(rcl, local, trace, false, args,
+                                            dp_grid(c, b - a, rcl, local, trace,
                                                     false),
                                             s));
                     }
@@ -1073,21 +1077,23 @@ int pa_get_kernel_stats(pa_ctx *c, pa_kernel_stat *out, int cap, int *n)
     static const char *mname[3] = {"sw", "swt", "nwt"};
     int k = 0;
     for (int m = 0; m < 3; ++m)
-        for (int b = 0; b < kBins + 3; ++b) {
+        for (int b = 0; b < kBins + kClasses; ++b) {
             const auto &ks = c->kstat[m][b];
             if (!ks.launches)
                 continue;
             if (k < cap) {
                 pa_kernel_stat &o = out[k];
                 std::memset(&o, 0, sizeof(o));
+                // classes: 16 lanes x 4 / 8 rows, 64 lanes x 4 / 8 rows
+                static const char *cname[kClasses] = {"l16r4", "l16r8", "r4",
+                                                      "r8"};
                 if (b < kBins)
-                    std::snprintf(o.name, sizeof(o.name), "%s.r%d.%s",
-                                  mname[m], b / 2 == 0 ? 2 : (b / 2 == 1 ? 4 : 8),
+                    std::snprintf(o.name, sizeof(o.name), "%s.%s.%s",
+                                  mname[m], cname[b / 2],
                                   b % 2 ? "i32" : "i16");
                 else
-                    std::snprintf(o.name, sizeof(o.name), "%s.r%d.rerun",
-                                  mname[m],
-                                  b - kBins == 0 ? 2 : (b - kBins == 1 ? 4 : 8));
+                    std::snprintf(o.name, sizeof(o.name), "%s.%s.rerun",
+                                  mname[m], cname[b - kBins]);
                 o.launches = ks.launches;
                 o.ms = ks.ms;
                 o.cells = b < kBins ? cells[m * kBins + b] : 0;

@@ -49,7 +49,7 @@ struct pa_ctx {
     struct KStat {
         uint64_t launches = 0;
         double ms = 0;
-    } kstat[3][pa::kBins + 3];        // DP mode x (bins, int32 re-runs per R)
+    } kstat[3][pa::kBins + pa::kClasses];   // DP mode x (bins, int32 re-runs)
     std::vector<Timer> pending;
     std::vector<hipEvent_t> free_events;
     pa_stats stats{};

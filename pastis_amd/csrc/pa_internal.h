@@ -9,6 +9,23 @@
 
 namespace pa {
 
+// Pointers into global memory.  Pointers that reach a kernel inside an
+// argument struct or as a stored address are generic, and generic accesses
+// compile to flat instructions, which also count against the LDS wait
+// counter; casting them to the global address space gives global_load /
+// global_store instead.
+#ifndef PA_GLOBAL   // the CPU emulation build of the tests defines it empty
+#define PA_GLOBAL __attribute__((address_space(1)))
+#endif
+template <class T> __device__ inline T PA_GLOBAL *glb(T *p)
+{
+    return (T PA_GLOBAL *)p;
+}
+template <class T> __device__ inline T PA_GLOBAL *glb_addr(uint64_t a)
+{
+    return (T PA_GLOBAL *)(uintptr_t)a;
+}
+
 // Residue codes 0..24 are real letters; code 25 is the sentinel used for the
 // DP's column 0 and for columns past the end of the streamed sequence.  Its
 // profile entries are -128, strictly below any real score.
@@ -105,10 +122,23 @@ __host__ __device__ inline uint64_t trace_rec(uint32_t qlen, uint32_t tlen,
     return (uint64_t)p * (tlen + 127u) * 64u + (uint64_t)(t - 1u) * nact + lane;
 }
 
-// Rows per lane chosen for a query length (256, 512 or 1024 rows per pass).
+// DP kernel class of a task by its row count:
+//   0: qlen <= 128, 16 lanes x 4 rows (four tasks per wave, one pass);
+//   1: qlen <= 256, 16 lanes x 8 rows (four tasks per wave, one pass);
+//   2: qlen <= 512, 64 lanes x 4 rows (one pass);
+//   3: longer,      64 lanes x 8 rows (1024 rows per pass).
+__host__ __device__ inline int dp_class(uint32_t qlen)
+{
+    return qlen <= 128 ? 0 : (qlen <= 256 ? 1 : (qlen <= 512 ? 2 : 3));
+}
+
+// Rows per lane of the class.  The 16-lane classes are single-pass, so their
+// trace records fall exactly where the 64-lane layout with the same R puts
+// them (no full pass; ceil(qlen / 2R) active lanes).
 __host__ __device__ inline int rows_per_lane(uint32_t qlen)
 {
-    return qlen <= 256 ? 2 : (qlen <= 512 ? 4 : 8);
+    const int c = dp_class(qlen);
+    return c == 0 || c == 2 ? 4 : 8;
 }
 
 // X-drop extension task: one seed of one pair.
@@ -133,7 +163,7 @@ struct XdOut {
 static_assert(sizeof(XdOut) == 40, "XdOut layout");
 
 // Launchers (k_dp.hip, k_xdrop.hip, k_misc.hip).
-hipError_t launch_dp(int R, bool local, bool trace, bool i16,
+hipError_t launch_dp(int cls, bool local, bool trace, bool i16,
                      const DpArgs &a, int grid, hipStream_t s);
 hipError_t launch_traceback(const TbArgs &a, int grid, hipStream_t s);
 // X-drop with one 16-lane row of a wave per work item (k_xdrop_coop.hip);
@@ -145,8 +175,9 @@ hipError_t launch_xdrop_coop(const XdTask *tasks, const uint32_t *list,
                              uint32_t *ovf_count, int grid, hipStream_t s);
 int xdrop_coop_occupancy();
 
-// Workgroups of one DP kernel variant that fit on one CU (occupancy).
-int dp_occupancy(int R, bool local, bool trace, bool i16);
+// Workgroups of one DP kernel variant (class as dp_class) that fit on one
+// CU (occupancy).
+int dp_occupancy(int cls, bool local, bool trace, bool i16);
 // Same for the LDS-ring X-drop kernel with `ring` slots per lane.
 int xdrop_occupancy(int ring);
 

@@ -26,6 +26,7 @@
 #define __launch_bounds__(...)
 #define __shared__ static
 #define __constant__ static const
+#define PA_GLOBAL   // host memory has one address space
 #define __forceinline__ inline
 #define HIP_DYNAMIC_SHARED(type, var) type *var = (type *)emu::dyn_shared();
 
@@ -144,7 +145,8 @@ inline float __int_as_float(int x)
     return f;
 }
 
-// DPP patterns the kernels use: wave_shr:1 (0x138) and row_ror:k
+// DPP patterns the kernels use: wave_shr:1 (0x138), row_shr:1 (0x111) and
+// row_ror:k
 // (0x121..0x12f, lane j of a 16-lane row reads lane (j - k) mod 16).
 inline int emu_update_dpp(int old, int src, int ctrl, int, int, bool)
 {
@@ -152,6 +154,10 @@ inline int emu_update_dpp(int old, int src, int ctrl, int, int, bool)
     if (ctrl == 0x138) {
         uint64_t v = emu::wave_exchange((uint32_t)src, lane - 1, false);
         return lane == 0 ? old : (int)(uint32_t)v;
+    }
+    if (ctrl == 0x111) {   // row_shr:1, the first lane of a row keeps old
+        uint64_t v = emu::wave_exchange((uint32_t)src, lane - 1, false);
+        return (lane & 15) == 0 ? old : (int)(uint32_t)v;
     }
     if (ctrl > 0x120 && ctrl <= 0x12f) {
         const int k = ctrl - 0x120;
