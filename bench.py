@@ -47,7 +47,7 @@ def host_cores():
     return max(1, n)
 
 
-def roofline(kstats, st, step_ms_total):
+def roofline(kstats, st, step_ms_total, tuning=None):
     """Dominant kernel by device time: algorithmic ops / launch time.
 
     Candidates are the DP kernel classes (k_dp instantiations) and the X-drop
@@ -62,7 +62,10 @@ def roofline(kstats, st, step_ms_total):
                           peak=PEAK_I16_TOPS if k["name"].endswith("i16")
                           else PEAK_I32_TOPS))
     if st.get("ms_xdrop", 0) > 0:
-        cands.append(dict(kernel="k_xdrop_lds", ms=st["ms_xdrop"],
+        tn = tuning or {}
+        xk = ("k_xdrop_coop + k_xdrop_pl<%d>" % tn.get("xd_ring", 16)
+              if tn.get("xd_coop") else "k_xdrop_pl<%d>" % tn.get("xd_ring", 16))
+        cands.append(dict(kernel=xk, ms=st["ms_xdrop"],
                           launches=None, cells=st["xdrop_cells"], ops=7,
                           peak=PEAK_I32_TOPS))
     if not cands:
@@ -327,7 +330,7 @@ def main():
         "xdrop_cells_per_step": int(st["xdrop_cells"] / args.steps),
         "passing_pairs": int(((out["flags"] & 1) != 0).sum())
         if args.workload == "xdrop" else None,
-        "roofline": roofline(ks, st, elapsed * 1e3),
+        "roofline": roofline(ks, st, elapsed * 1e3, tuning),
         "device_ms_per_step": {k: round(st[k] / args.steps, 3) for k in
                                ("ms_sw", "ms_xdrop", "ms_nw", "ms_trace")},
         "cpu_baseline": None,

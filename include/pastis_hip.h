@@ -280,9 +280,9 @@ typedef struct pa_tuning {
                                1..32; 0 = as many as occupancy allows */
     int32_t xd_wg_per_cu;   /* X-drop workgroups (one wave) per CU, 0 = max */
     int32_t xd_ring;        /* X-drop LDS ring slots per lane: 16 or 32 */
-    int32_t xd_perlane;     /* 0: X-drop items go to the 16-lanes-per-item
+    int32_t xd_coop;        /* 1: X-drop items go to the 16-lanes-per-item
                                kernel first (band up to 14 columns), the rest
-                               to the one-lane-per-item kernels; 1: one lane
+                               to the one-lane-per-item kernels; 0: one lane
                                per item only */
 } pa_tuning;
 

@@ -76,7 +76,7 @@ class PaKernelStat(C.Structure):
 
 class PaTuning(C.Structure):
     _fields_ = [("dp_wg_per_cu", C.c_int32), ("xd_wg_per_cu", C.c_int32),
-                ("xd_ring", C.c_int32), ("xd_perlane", C.c_int32)]
+                ("xd_ring", C.c_int32), ("xd_coop", C.c_int32)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

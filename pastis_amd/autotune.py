@@ -25,21 +25,22 @@ def candidates(mode_xdrop=True):
     if not mode_xdrop:
         for dp in DP_WG:
             yield dict(dp_wg_per_cu=dp, xd_wg_per_cu=0, xd_ring=16,
-                       xd_perlane=0)
+                       xd_coop=0)
         return
     for dp, ring in itertools.product(DP_WG, XD_RING):
-        yield dict(dp_wg_per_cu=dp, xd_wg_per_cu=0, xd_ring=ring,
-                   xd_perlane=0)
+        yield dict(dp_wg_per_cu=dp, xd_wg_per_cu=0, xd_ring=ring, xd_coop=1)
     for xd, ring in itertools.product(XD_WG, XD_RING):
-        yield dict(dp_wg_per_cu=0, xd_wg_per_cu=xd, xd_ring=ring,
-                   xd_perlane=1)
+        yield dict(dp_wg_per_cu=0, xd_wg_per_cu=xd, xd_ring=ring, xd_coop=0)
 
 
-def tune(al, pairs, xdrop=True, sample=200000, repeats=2, seed=0,
+def tune(al, pairs, xdrop=True, sample=1000000, repeats=2, seed=0,
          cache=None, log=None):
     """Picks the fastest tuning for `al` on a sample of `pairs`, installs it
-    and returns (best, table).  cache: optional JSON file keyed by the
-    sample's size and mode."""
+    and returns (best, table).  The X-drop settings are chosen by the X-drop
+    kernels' device time and the DP setting by the DP kernels' device time
+    (each measured with the events of pa_get_stats), so that one stage's
+    noise does not decide the other's.  cache: optional JSON file keyed by
+    the sample's size and mode."""
     key = "%s:%d" % ("xdrop" if xdrop else "sw", min(sample, len(pairs)))
     if cache and os.path.exists(cache):
         with open(cache) as fh:
@@ -63,21 +64,30 @@ def tune(al, pairs, xdrop=True, sample=200000, repeats=2, seed=0,
             al.set_tuning(**cand)
             run(dp, n, do)          # warm-up (first launch, allocations)
             al.synchronize()
-            best = np.inf
+            best = dict(ms=np.inf, ms_dp=np.inf, ms_xdrop=np.inf)
             for _ in range(repeats):
+                al.reset_stats()
                 t = time.perf_counter()
                 run(dp, n, do)
                 al.synchronize()
-                best = min(best, time.perf_counter() - t)
-            table.append(dict(cand, ms=round(best * 1e3, 3)))
+                el = time.perf_counter() - t
+                st = al.stats()
+                best["ms"] = min(best["ms"], el * 1e3)
+                best["ms_dp"] = min(best["ms_dp"], st["ms_nw"] + st["ms_sw"])
+                best["ms_xdrop"] = min(best["ms_xdrop"], st["ms_xdrop"])
+            table.append(dict(cand, **{k: round(v, 3) for k, v in best.items()}))
             if log:
-                log("autotune %s: %.3f ms" % (cand, best * 1e3))
+                log("autotune %s: %.3f ms (dp %.3f, xdrop %.3f)" %
+                    (cand, best["ms"], best["ms_dp"], best["ms_xdrop"]))
     finally:
         al.device_free(dp)
         al.device_free(do)
-    best = min(table, key=lambda r: r["ms"])
-    choice = {k: best[k] for k in ("dp_wg_per_cu", "xd_wg_per_cu", "xd_ring",
-                                   "xd_perlane")}
+    # the DP grid from the DP time, the X-drop shape from the X-drop time
+    bdp = min(table, key=lambda r: r["ms_dp"])
+    bxd = min(table, key=lambda r: r["ms_xdrop"]) if xdrop else bdp
+    choice = {"dp_wg_per_cu": bdp["dp_wg_per_cu"],
+              "xd_wg_per_cu": bxd["xd_wg_per_cu"], "xd_ring": bxd["xd_ring"],
+              "xd_coop": bxd["xd_coop"]}
     al.set_tuning(**choice)
     if cache:
         known = {}

@@ -440,7 +440,7 @@ int run_xdrop(pa_ctx *c, const pa_pair *d_pairs, uint64_t n, hipStream_t s)
     {
         Scope t(c, 1, s);
         const int grid = xd_grid(c, ni);
-        if (!c->tune.xd_perlane) {
+        if (c->tune.xd_coop) {
             // 16 lanes per item first; items with a wider band go to the
             // one-lane-per-item kernel through the `lng` list
             const int cg = (int)std::max<uint64_t>(
@@ -760,8 +760,8 @@ int pa_set_tuning(pa_ctx *c, const pa_tuning *t)
         t->xd_wg_per_cu < 0 || t->xd_wg_per_cu > kMaxWgPerCu ||
         (t->xd_ring != 16 && t->xd_ring != 32))
         return fail(c, PA_ERR_ARG, "tuning outside its range");
-    if (t->xd_perlane != 0 && t->xd_perlane != 1)
-        return fail(c, PA_ERR_ARG, "xd_perlane must be 0 or 1");
+    if (t->xd_coop != 0 && t->xd_coop != 1)
+        return fail(c, PA_ERR_ARG, "xd_coop must be 0 or 1");
     c->tune = *t;
     return PA_OK;
 }

@@ -108,9 +108,9 @@ def test_results_do_not_depend_on_tuning():
                                  (32, 3, 2, 0), (16, 0, 0, 1)]:
             al = make_aligner(model)
             al.set_tuning(xd_ring=ring, dp_wg_per_cu=dp, xd_wg_per_cu=xd,
-                          xd_perlane=pl)
+                          xd_coop=1 - pl)
             assert al.tuning() == dict(dp_wg_per_cu=dp, xd_wg_per_cu=xd,
-                                       xd_ring=ring, xd_perlane=pl)
+                                       xd_ring=ring, xd_coop=1 - pl)
             al.load_seqs(w.codes, w.offsets)
             al.reset_stats()
             got = al.align(w.pairs)
