@@ -187,6 +187,16 @@ int pa_sw_scores(struct pa_ctx *ctx, const pa_pair *pairs, uint64_t n,
 int pa_sw_scores_dev(struct pa_ctx *ctx, const pa_pair *d_pairs, uint64_t n,
                      pa_score *d_out, void *stream);
 
+/* pa_sw_scores_dev as a replayed HIP graph: the first call with a given
+ * (d_pairs, n, d_out, stream) captures the batch's kernels into a graph,
+ * later calls replay it.  The pairs' contents may change between calls;
+ * their values must stay valid for the loaded sequences (the graph does not
+ * re-validate).  A different (d_pairs, n, d_out, stream) recaptures.
+ * Replaces the per-batch launches behind pw_aln_batch's loop
+ * (src/DistPWRunner.cpp:116-131) for score-only batches of equal size. */
+int pa_sw_scores_graph(struct pa_ctx *ctx, const pa_pair *d_pairs, uint64_t n,
+                       pa_score *d_out, void *stream);
+
 /* Seed extension only: extendSeed(..., GappedXDrop) of
  * SeqanXdropAligner.cpp:279-289 for seed `seed` (0 or 1) of every pair. */
 int pa_xdrop_extents(struct pa_ctx *ctx, const pa_pair *pairs, uint64_t n,

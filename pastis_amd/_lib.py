@@ -30,7 +30,7 @@ EXPORTS = [
     "pa_filter_set_model", "pa_filter_get_model", "pa_filter_grad",
     "pa_filter_train", "pa_filter_logits", "pa_filter_score_dev",
     "pa_filter_select_dev", "pa_set_thresholds", "pa_get_tuning",
-    "pa_set_tuning",
+    "pa_set_tuning", "pa_sw_scores_graph",
 ]
 
 PA_FILTER_NF = 16
@@ -121,7 +121,7 @@ def lib():
     L.pa_load_seqs.argtypes = [vp, i32, vp, vp, u64, i32]
     for f in ("pa_align", "pa_sw_scores"):
         getattr(L, f).argtypes = [vp, vp, u64, vp]
-    for f in ("pa_align_dev", "pa_sw_scores_dev"):
+    for f in ("pa_align_dev", "pa_sw_scores_dev", "pa_sw_scores_graph"):
         getattr(L, f).argtypes = [vp, vp, u64, vp, vp]
     L.pa_xdrop_extents.argtypes = [vp, vp, u64, i32, vp]
     L.pa_kmer_prune.argtypes = [vp, u64, i32, C.c_float, vp,

@@ -109,12 +109,18 @@ hipEvent_t get_event(pa_ctx *c)
     return e;
 }
 
+// Event timer of one region on a stream; inactive while the context's
+// stream is being captured into a graph.
 struct Scope {
     pa_ctx *c;
     Timer t;
     hipStream_t s;
-    Scope(pa_ctx *c_, int kind, hipStream_t s_) : c(c_), s(s_)
+    bool on;
+    Scope(pa_ctx *c_, int kind, hipStream_t s_)
+        : c(c_), s(s_), on(!c_->capturing)
     {
+        if (!on)
+            return;
         t.kind = kind;
         t.a = get_event(c);
         t.b = get_event(c);
@@ -122,6 +128,8 @@ struct Scope {
     }
     ~Scope()
     {
+        if (!on)
+            return;
         hipEventRecord(t.b, s);
         c->pending.push_back(t);
     }
@@ -210,7 +218,8 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
     int rc;
     if ((rc = ensure_bnd(c)))
         return rc;
-    HIPCHK(c, hipMemsetAsync(c->tflags.p, 0, nt, s));
+    // a fill kernel, not a memset node, so that the sequence can be captured
+    HIPCHK(c, fill_u32(P<uint32_t>(c->tflags), 0, (uint32_t)((nt + 3) / 4), s));
     std::vector<std::pair<uint64_t, uint64_t>> chunks;
     if (trace) {
         if ((rc = grow(c, c->sizes, nt * sizeof(uint64_t) + 8)))
@@ -723,6 +732,10 @@ void pa_destroy(pa_ctx *c)
     if (c->st)
         hipStreamSynchronize(c->st);
     resolve_timers(c);
+    if (c->gexec)
+        hipGraphExecDestroy(c->gexec);
+    if (c->graph)
+        hipGraphDestroy(c->graph);
     if (c->shared_cols) {
         c->ccodes = Buf();
         c->coffs = Buf();
@@ -893,6 +906,64 @@ int pa_sw_scores_dev(pa_ctx *c, const pa_pair *d_pairs, uint64_t n,
     if ((rc = run_dp(c, n, true, false, 0, s)))
         return rc;
     HIPCHK(c, scores_out(P<DpOut>(c->dpout), n, d_out, s));
+    c->stats.pairs += n;
+    return PA_OK;
+}
+
+int pa_sw_scores_graph(pa_ctx *c, const pa_pair *d_pairs, uint64_t n,
+                       pa_score *d_out, void *stream)
+{
+    int rc;
+    if ((rc = check_ready(c)))
+        return rc;
+    if (n == 0)
+        return PA_OK;
+    if (!d_pairs || !d_out)
+        return fail(c, PA_ERR_ARG, "null pointer");
+    if (n > 0xffffffffull)
+        return fail(c, PA_ERR_LIMIT, "more than 2^32 pairs");
+    hipSetDevice(c->device);
+    hipStream_t s = pick(c, stream);
+    if (!c->gexec || c->g_pairs != d_pairs || c->g_n != n || c->g_out != d_out ||
+        c->g_stream != s) {
+        if (c->gexec)
+            hipGraphExecDestroy(c->gexec);
+        if (c->graph)
+            hipGraphDestroy(c->graph);
+        c->gexec = nullptr;
+        c->graph = nullptr;
+        // every buffer the batch uses exists before the capture starts
+        if ((rc = ensure_common(c, n)) || (rc = ensure_bnd(c)))
+            return rc;
+        HIPCHK(c, hipStreamSynchronize(s));
+        HIPCHK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        c->capturing = true;
+        rc = build_sw_tasks(d_pairs, n, P<uint64_t>(c->roffs),
+                            P<uint64_t>(c->coffs), P<DpTask>(c->tasks),
+                            (unsigned long long *)P<uint64_t>(c->cells), s) !=
+                     hipSuccess
+                 ? PA_ERR_HIP
+                 : PA_OK;
+        if (!rc)
+            rc = run_dp(c, n, true, false, 0, s);
+        if (!rc && scores_out(P<DpOut>(c->dpout), n, d_out, s) != hipSuccess)
+            rc = PA_ERR_HIP;
+        c->capturing = false;
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(s, &g);
+        if (rc || e != hipSuccess) {
+            if (g)
+                hipGraphDestroy(g);
+            return fail(c, rc ? rc : PA_ERR_HIP, "graph capture failed");
+        }
+        c->graph = g;
+        HIPCHK(c, hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
+        c->g_pairs = d_pairs;
+        c->g_n = n;
+        c->g_out = d_out;
+        c->g_stream = s;
+    }
+    HIPCHK(c, hipGraphLaunch(c->gexec, s));
     c->stats.pairs += n;
     return PA_OK;
 }

@@ -55,6 +55,13 @@ struct pa_ctx {
     pa_stats stats{};
     pa::FilterState *flt = nullptr;
     int ncu = 256;                // compute units of the device
+    // hipGraph of one score-only SW batch (pa_sw_scores_graph)
+    bool capturing = false;       // no event timers while a stream captures
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    const void *g_pairs = nullptr, *g_out = nullptr;
+    uint64_t g_n = 0;
+    hipStream_t g_stream = nullptr;
     pa_tuning tune{0, 0, 16, 0};
 };
 

@@ -223,6 +223,24 @@ hipError_t hipEventElapsedTime(float *, hipEvent_t, hipEvent_t);
 hipError_t hipEventDestroy(hipEvent_t);
 hipError_t hipGetDeviceCount(int *);
 hipError_t hipGetDeviceProperties(hipDeviceProp_t *, int);
+// graph capture is not emulated: the calls report failure
+typedef void *hipGraph_t;
+typedef void *hipGraphExec_t;
+enum { hipStreamCaptureModeThreadLocal = 1 };
+inline hipError_t hipStreamBeginCapture(hipStream_t, int) { return 1; }
+inline hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t *g)
+{
+    *g = nullptr;
+    return 1;
+}
+inline hipError_t hipGraphInstantiate(hipGraphExec_t *, hipGraph_t, void *,
+                                      void *, size_t)
+{
+    return 1;
+}
+inline hipError_t hipGraphLaunch(hipGraphExec_t, hipStream_t) { return 1; }
+inline hipError_t hipGraphDestroy(hipGraph_t) { return 0; }
+inline hipError_t hipGraphExecDestroy(hipGraphExec_t) { return 0; }
 template <class F>
 hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int *n, F, int, size_t)
 {
