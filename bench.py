@@ -140,15 +140,20 @@ def cpu_baseline_sw(w, pairs, go, ge, budget_s, threads):
 
 
 def learned_filter(args, cfg, al, d_pairs, n, plain_out, plain_ms):
-    """Train the prefilter on GPU-aligner labels of a separate config-2
-    workload (2M pairs at scale 1, generator seed 202), calibrate tau for
-    99.5 % recall on a held-out tenth, then run filter + exact aligner on the
-    benchmark pairs and compare with the plain step."""
+    """Train the prefilter on GPU-aligner labels of a separate workload of
+    the same generator (X-drop: config 2 with 2M pairs at scale 1, seed 202;
+    SW: config 1, seed 201), calibrate tau for 99.5 % recall on a held-out
+    tenth, then run filter + exact aligner on the benchmark pairs and compare
+    with the plain operator (pa_align on every pair)."""
     from pastis_amd import prefilter, synth
     from pastis_amd._lib import PAIR_DTYPE, RESULT_DTYPE
     from pastis_amd.aligner import Aligner
     t = time.time()
-    tw = synth.config2(scale=0.4 * args.scale, seed=202, threads=host_cores())
+    if args.workload == "xdrop":
+        tw = synth.config2(scale=0.4 * args.scale, seed=202,
+                           threads=host_cores())
+    else:
+        tw = synth.config1(scale=args.scale, seed=201, threads=host_cores())
     tal = Aligner(cfg)
     tal.load_seqs(tw.codes, tw.offsets)
     y = prefilter.labels_of(tal.align(tw.pairs))
@@ -221,7 +226,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["xdrop", "sw"], default="xdrop")
+    ap.add_argument("--workload", choices=["xdrop", "sw", "sw3"],
+                    default="xdrop")
+    ap.add_argument("--graph", action="store_true",
+                    help="sw / sw3: replay each step from a captured HIP graph")
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--gap-model", default="as_called",
                     choices=["as_called", "as_documented"])
@@ -258,8 +266,11 @@ def main():
     if args.workload == "xdrop":
         w = synth.config2(scale=args.scale, threads=host_cores())
         cfg = config_for_cli(-11, -2, 6, 49, 2, args.gap_model)
-    else:
+    elif args.workload == "sw":
         w = synth.config1(scale=args.scale, threads=host_cores())
+        cfg = config_for_cli(-11, -2, 6, 0, 2, args.gap_model)
+    else:
+        w = synth.config3(scale=args.scale, threads=host_cores())
         cfg = config_for_cli(-11, -2, 6, 0, 2, args.gap_model)
     pairs = w.pairs
     n = len(pairs)
@@ -285,6 +296,8 @@ def main():
     def step():
         if args.workload == "xdrop":
             al.align_dev(d_pairs, n, d_out)
+        elif args.graph:
+            al.sw_scores_graph(d_pairs, n, d_out)
         else:
             al.sw_scores_dev(d_pairs, n, d_out)
 
@@ -338,6 +351,24 @@ def main():
     if args.workload == "xdrop" and not args.no_filter:
         line["learned_filter"] = learned_filter(args, cfg, al, d_pairs, n, out,
                                                 ms_per_step)
+    elif args.workload == "sw" and not args.no_filter:
+        # the plain operator: local alignment with traceback statistics and
+        # the coverage/identity predicate on every pair
+        d_res = al.device_alloc(n * RESULT_DTYPE.itemsize)
+        for _ in range(args.warmup):
+            al.align_dev(d_pairs, n, d_res)
+        al.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            al.align_dev(d_pairs, n, d_res)
+        al.synchronize()
+        op_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        res = np.zeros(n, RESULT_DTYPE)
+        al.d2h(res, d_res)
+        al.device_free(d_res)
+        lf = learned_filter(args, cfg, al, d_pairs, n, res, op_ms)
+        lf["plain_operator_ms_per_step"] = round(op_ms, 3)
+        line["learned_filter"] = lf
     if not args.no_cpu:
         threads = host_cores()
         if args.workload == "xdrop":
