@@ -11,7 +11,7 @@
 // sequences take several passes linked by a boundary row in global memory.
 //
 // The substitution scores come from a query profile in LDS: for each residue
-// code c and each row, score(row residue, c) + 128 as a byte.
+// code c and each row, score(row residue, c) as an int8.
 //
 // Recurrences (Gotoh; rows i over the row sequence, columns j over the
 // column sequence; gap of length L costs open + (L-1)*ext):
@@ -23,14 +23,26 @@
 // then smaller row), as SeqAn's column-wise fill with a strict '>' update.
 //
 // Traced kernels work on 4*value + tag so that one max picks the value and
-// the predecessor at once (tags: 3 restart, 2 diagonal, 1 F, 0 E); they store
-// a 4-bit code per cell (source, E extended, F extended) for the traceback.
+// the predecessor at once (tags: 3 restart (local) or diagonal between equal
+// residues (global), 2 diagonal, 1 F, 0 E); they store
+// a 4-bit code per cell (bits 0-1 source, bit 2 F extended, bit 3 E
+// extended) for the traceback.
 #include "pa_internal.h"
+
+#include <type_traits>
 
 namespace pa {
 namespace {
 
 typedef short s2v __attribute__((ext_vector_type(2)));
+
+// v_perm selectors that turn byte 1 (byte 3) of lw and of hw into two
+// sign-extended int16 halves: byte 8 / 9 of a selector repeats the sign bit
+// of byte 1 / 3 of the low source, 10 / 11 of the high source.
+constexpr uint32_t kSelB1 = 0x0a050801u;
+constexpr uint32_t kSelB3 = 0x0b070903u;
+// bytes 0 and 2 of the low source, then bytes 0 and 2 of the high source
+constexpr uint32_t kSelEven = 0x06040200u;
 
 template <bool I16> struct Ops;
 
@@ -69,24 +81,35 @@ template <> struct Ops<true> {
         uint32_t mm = ((uint32_t)m & 0xffffu) * 0x10001u;
         return v(u(a) | mm);
     }
+    // a | 1 in both halves: sets the F predecessor tag of a traced F value
+    // whose low two bits are 00 or 01
+    __device__ static inline V fmark(V a) { return v(u(a) | 0x00010001u); }
     __device__ static inline int lo(V a) { return a.x; }
     __device__ static inline int hi(V a) { return a.y; }
-    template <int K>
-    __device__ static inline V prof(uint32_t lw, uint32_t hw)
+    // profile rows 4w..4w+3 of the lo strip (bytes of lw) and of the hi
+    // strip (bytes of hw), sign-extended to int16 pairs
+    template <int R>
+    __device__ static inline void unpack(const uint32_t *lw, const uint32_t *hw,
+                                         V *s)
     {
-        // byte K of lw -> low half, byte K of hw -> high half, zero-extended
-        const uint32_t sel = (uint32_t)K | (0x0cu << 8) |
-                             ((uint32_t)(4 + K) << 16) | (0x0cu << 24);
-        uint32_t x = __builtin_amdgcn_perm(hw, lw, sel);
-        return v(x) + splat(-128);
+#pragma unroll
+        for (int w = 0; w < R / 4; ++w) {
+            const uint32_t l = lw[w], h = hw[w], l8 = l << 8, h8 = h << 8;
+            s[4 * w + 0] = v(__builtin_amdgcn_perm(h8, l8, kSelB1));
+            s[4 * w + 1] = v(__builtin_amdgcn_perm(h, l, kSelB1));
+            s[4 * w + 2] = v(__builtin_amdgcn_perm(h8, l8, kSelB3));
+            s[4 * w + 3] = v(__builtin_amdgcn_perm(h, l, kSelB3));
+        }
     }
     // new row-above input: lo <- hi of lane-1, hi <- own lo (CTRL: DPP
-    // wave_shr:1 for 64-lane tasks, row_shr:1 for 16-lane tasks)
-    template <int CTRL> __device__ static inline V shuffle_up(V out)
+    // wave_shr:1 for 64-lane tasks, row_shr:1 for 16-lane tasks).  The
+    // first lane of a task has no source lane and keeps the DPP "old"
+    // operand, which carries `first` (the boundary value) in its high half.
+    template <int CTRL> __device__ static inline V shuffle_up(V out, int first)
     {
         uint32_t o = u(out);
         uint32_t sh = (uint32_t)__builtin_amdgcn_update_dpp(
-            0, (int)o, CTRL, 0xf, 0xf, false);
+            (int)((uint32_t)first << 16), (int)o, CTRL, 0xf, 0xf, false);
         return v(__builtin_amdgcn_alignbit(o, sh, 16));
     }
     __device__ static inline V set_lo(V a, int x)
@@ -94,17 +117,36 @@ template <> struct Ops<true> {
         a.x = (short)x;
         return a;
     }
-    // bits 0-3 of each half: the code nibble of that half
-    __device__ static inline uint32_t code(V hp, V ep, V fp)
-    {
-        return (u(hp) & 0x00030003u) | ((u(ep) & 0x00010001u) << 2) |
-               ((u(fp) & 0x00010001u) << 3);
-    }
     // replace the halves selected by (ilo, ihi) of old with those of nw
     __device__ static inline V blend(V old, V nw, bool ilo, bool ihi)
     {
         uint32_t m = (ilo ? 0x0000ffffu : 0u) | (ihi ? 0xffff0000u : 0u);
         return v((u(nw) & m) | (u(old) & ~m));
+    }
+    // F-extension tag (bit 0 of fp) into bit 0, E-extension tag (bit 1 of
+    // ep) kept in bit 1; higher bits are don't-care
+    __device__ static inline V efbits(V fp, V ep)
+    {
+        return v((u(fp) & 0x00010001u) | (u(ep) & 0xfffefffeu));
+    }
+    // Trace record of R rows x 2 halves.  Code nibble of a cell: bits 0-1
+    // predecessor tag, bit 2 F extended, bit 3 E extended.  Row r, half h
+    // sits in byte 4*(r/4) + 2*(r%2) + h, high nibble when (r/2) is odd.
+    template <int R>
+    __device__ static inline void pack(const V *hp, const V *ef, uint32_t *w)
+    {
+        uint32_t y[R / 2];
+#pragma unroll
+        for (int q = 0; q < R / 2; ++q) {
+            const uint32_t px =
+                __builtin_amdgcn_perm(u(ef[2 * q + 1]), u(ef[2 * q]), kSelEven);
+            const uint32_t ph =
+                __builtin_amdgcn_perm(u(hp[2 * q + 1]), u(hp[2 * q]), kSelEven);
+            y[q] = (ph & 0x03030303u) | ((px << 2) & 0xfcfcfcfcu);
+        }
+#pragma unroll
+        for (int k = 0; k < R / 4; ++k)
+            w[k] = (y[2 * k] & 0x0f0f0f0fu) | ((y[2 * k + 1] << 4) & 0xf0f0f0f0u);
     }
     __device__ static inline Key key(int score, int col)
     {
@@ -136,17 +178,22 @@ template <> struct Ops<false> {
     }
     __device__ static inline V andm(V a, int m) { return I2{a.x & m, a.y & m}; }
     __device__ static inline V orm(V a, int m) { return I2{a.x | m, a.y | m}; }
+    __device__ static inline V fmark(V a) { return I2{a.x | 1, a.y | 1}; }
     __device__ static inline int lo(V a) { return a.x; }
     __device__ static inline int hi(V a) { return a.y; }
-    template <int K>
-    __device__ static inline V prof(uint32_t lw, uint32_t hw)
+    template <int R>
+    __device__ static inline void unpack(const uint32_t *lw, const uint32_t *hw,
+                                         V *s)
     {
-        return I2{(int)((lw >> (8 * K)) & 0xffu) - 128,
-                  (int)((hw >> (8 * K)) & 0xffu) - 128};
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            s[r] = I2{(int)(int8_t)(lw[r / 4] >> (8 * (r % 4))),
+                      (int)(int8_t)(hw[r / 4] >> (8 * (r % 4)))};
     }
-    template <int CTRL> __device__ static inline V shuffle_up(V out)
+    template <int CTRL> __device__ static inline V shuffle_up(V out, int first)
     {
-        int sh = __builtin_amdgcn_update_dpp(0, out.y, CTRL, 0xf, 0xf, false);
+        int sh = __builtin_amdgcn_update_dpp(first, out.y, CTRL, 0xf, 0xf,
+                                             false);
         return I2{sh, out.x};
     }
     __device__ static inline V set_lo(V a, int x)
@@ -154,17 +201,30 @@ template <> struct Ops<false> {
         a.x = x;
         return a;
     }
-    __device__ static inline uint32_t code(V hp, V ep, V fp)
-    {
-        uint32_t lo = ((uint32_t)hp.x & 3u) | (((uint32_t)ep.x & 1u) << 2) |
-                      (((uint32_t)fp.x & 1u) << 3);
-        uint32_t hi = ((uint32_t)hp.y & 3u) | (((uint32_t)ep.y & 1u) << 2) |
-                      (((uint32_t)fp.y & 1u) << 3);
-        return lo | (hi << 16);
-    }
     __device__ static inline V blend(V old, V nw, bool ilo, bool ihi)
     {
         return I2{ilo ? nw.x : old.x, ihi ? nw.y : old.y};
+    }
+    __device__ static inline V efbits(V fp, V ep)
+    {
+        return I2{(fp.x & 1) | (ep.x & ~1), (fp.y & 1) | (ep.y & ~1)};
+    }
+    // same record layout as the int16 kernels
+    template <int R>
+    __device__ static inline void pack(const V *hp, const V *ef, uint32_t *w)
+    {
+#pragma unroll
+        for (int k = 0; k < R / 4; ++k)
+            w[k] = 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const uint32_t clo = ((uint32_t)hp[r].x & 3u) |
+                                 (((uint32_t)ef[r].x & 3u) << 2);
+            const uint32_t chi = ((uint32_t)hp[r].y & 3u) |
+                                 (((uint32_t)ef[r].y & 3u) << 2);
+            const int sh = 8 * (2 * (r % 2)) + 4 * ((r / 2) % 2);
+            w[r / 4] |= (clo << sh) | (chi << (sh + 8));
+        }
     }
     __device__ static inline Key key(int score, int col)
     {
@@ -179,58 +239,22 @@ template <> struct Ops<false> {
     static constexpr int kNegTraced = -(1 << 28);
 };
 
-template <int R> struct TraceWord;
-template <> struct TraceWord<2> {
-    typedef uint16_t T;
-    __device__ static inline T pack(const uint32_t *c)
-    {
-        uint32_t a = c[0] | (c[1] << 4);
-        return (T)((a & 0xffu) | ((a >> 8) & 0xff00u));
-    }
-};
-template <> struct TraceWord<4> {
-    typedef uint32_t T;
-    __device__ static inline T pack(const uint32_t *c)
-    {
-        return c[0] | (c[1] << 4) | (c[2] << 8) | (c[3] << 12);
-    }
-};
-template <> struct TraceWord<8> {
-    typedef uint2 T;
-    __device__ static inline T pack(const uint32_t *c)
-    {
-        uint32_t a = c[0] | (c[1] << 4) | (c[2] << 8) | (c[3] << 12);
-        uint32_t b = c[4] | (c[5] << 4) | (c[6] << 8) | (c[7] << 12);
-        return make_uint2((a & 0xffffu) | (b << 16),
-                          (a >> 16) | (b & 0xffff0000u));
-    }
-};
-
 // One trace record (R bytes) to global memory.
-__device__ inline void store_word(uint8_t PA_GLOBAL *p, uint16_t w)
+__device__ inline void store_rec(uint8_t PA_GLOBAL *p, const uint32_t *w,
+                                 int nw)
 {
-    *(uint16_t PA_GLOBAL *)p = w;
-}
-__device__ inline void store_word(uint8_t PA_GLOBAL *p, uint32_t w)
-{
-    *(uint32_t PA_GLOBAL *)p = w;
-}
-__device__ inline void store_word(uint8_t PA_GLOBAL *p, uint2 w)
-{
-    typedef uint32_t v2u __attribute__((ext_vector_type(2)));
-    v2u x;
-    x.x = w.x;
-    x.y = w.y;
-    *(v2u PA_GLOBAL *)p = x;
+    if (nw == 1) {
+        *(uint32_t PA_GLOBAL *)p = w[0];
+    } else {
+        typedef uint32_t v2u __attribute__((ext_vector_type(2)));
+        v2u x;
+        x.x = w[0];
+        x.y = w[1];
+        *(v2u PA_GLOBAL *)p = x;
+    }
 }
 
 template <int R> struct ProfWords;
-template <> struct ProfWords<2> {
-    __device__ static inline void load(const uint8_t *p, uint32_t *w)
-    {
-        w[0] = *(const uint16_t *)p;
-    }
-};
 template <> struct ProfWords<4> {
     __device__ static inline void load(const uint8_t *p, uint32_t *w)
     {
@@ -243,24 +267,6 @@ template <> struct ProfWords<8> {
         uint2 x = *(const uint2 *)p;
         w[0] = x.x;
         w[1] = x.y;
-    }
-};
-
-template <class O, int R, int K>
-struct ProfUnpack {
-    __device__ static inline void run(const uint32_t *lw, const uint32_t *hw,
-                                      typename O::V *s)
-    {
-        ProfUnpack<O, R, K - 1>::run(lw, hw, s);
-        s[K - 1] = O::template prof<(K - 1) % 4>(lw[(K - 1) / 4],
-                                                 hw[(K - 1) / 4]);
-    }
-};
-template <class O, int R>
-struct ProfUnpack<O, R, 0> {
-    __device__ static inline void run(const uint32_t *, const uint32_t *,
-                                      typename O::V *)
-    {
     }
 };
 
@@ -285,6 +291,7 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
     typedef typename O::V V;
     typedef typename O::Key Key;
     static_assert(L == 64 || L == 16, "lanes per task");
+    static_assert(R == 4 || R == 8, "rows per lane");
     constexpr int G = 64 / L;                 // tasks per wave
     constexpr int ROWS = 2 * R * L;           // rows per pass
     constexpr int SHR = L == 64 ? 0x138 : 0x111;   // wave_shr:1 / row_shr:1
@@ -297,20 +304,23 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
     // query profile of this lane's task
     uint32_t *gprof32 = prof32 + grp * (kProfChars * ROWS / 4);
     const uint8_t *prof = (const uint8_t *)gprof32;
-    // biased score matrix (score or 4*score+2, plus 128) after the profiles;
-    // columns >= 25 and row 31 (padding rows) hold 0, i.e. -128
+    // score matrix (score, or 4*score+2 when traced, as int8) after the
+    // profiles; columns >= 25 and rows >= 25 (row 31: padding rows) hold
+    // -128.  Global traces tag a diagonal step between equal residues 3
+    // instead of 2 (4*score+3): the traceback counts identities from the
+    // tags, and diagonal still wins every tie against F and E.
     uint32_t *mb32 = prof32 + G * kProfChars * ROWS / 4;
     for (int w = lane; w < 256; w += 64) {
         uint32_t x = 0;
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
             const int q = w / 8, c = (w % 8) * 4 + b;
-            int val = 0;
+            int val = -128;
             if (q < kAlpha && c < kAlpha) {
                 const int sc = a.mat[q * 32 + c];
-                val = (TRACE ? 4 * sc + 2 : sc) + 128;
+                val = TRACE ? 4 * sc + 2 + (!LOCAL && q == c ? 1 : 0) : sc;
             }
-            x |= (uint32_t)val << (8 * b);
+            x |= ((uint32_t)val & 0xffu) << (8 * b);
         }
         mb32[w] = x;
     }
@@ -320,7 +330,12 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
     const int ext = a.gap_ext * (1 << SH);
     const int gmax = open > ext ? open : ext;
     const V OPEN = O::splat(open);
-    const V EXT = O::splat(TRACE ? ext + 1 : ext);
+    // Traced: E values carry the extension tag in bit 1 (+2), F values are
+    // carried with bit 0 set (fmark), so extending adds the plain 4*ext.
+    const V EXT = O::splat(ext);
+    const V EXTE = O::splat(TRACE ? ext + 2 : ext);
+    // F value of "no cell": NEG, tagged like every carried F when traced
+    constexpr int NEGF = TRACE ? NEG + 1 : NEG;
     int32_t PA_GLOBAL *bnd = glb(a.bnd) + (size_t)blockIdx.x * a.bnd_stride;
 
     // tasks claimed by this wave and not yet started: [cbase, cend)
@@ -426,23 +441,22 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
                 Ho[r] = O::add(H[r], OPEN);
             }
             V outH = H[R - 1];
-            V outF = LOCAL ? O::splat(NEG) : H[R - 1];
+            V outF = LOCAL ? O::splat(NEGF)
+                           : (TRACE ? O::fmark(H[R - 1]) : H[R - 1]);
             // inputs of step 1: row above at column t (up) and t-1 (diag)
             int bh0, bh1, bf1;
             if (p == 0) {
                 bh0 = 0;
                 bh1 = LOCAL ? 0 : open;
-                bf1 = NEG;
+                bf1 = NEGF;
             } else {
                 bh0 = bnd[0];
                 bh1 = bnd[2];
                 bf1 = bnd[3];
             }
-            V upH = O::template shuffle_up<SHR>(outH);
-            V upF = O::template shuffle_up<SHR>(outF);
+            V upH = O::template shuffle_up<SHR>(outH, bh1);
+            V upF = O::template shuffle_up<SHR>(outF, bf1);
             V dg = O::splat(NEG);
-            upH = gl == 0 ? O::set_lo(upH, bh1) : upH;
-            upF = gl == 0 ? O::set_lo(upF, bf1) : upF;
             dg = gl == 0 ? O::set_lo(dg, bh0) : dg;
             int c_lo = kSentinel, c_hi = kSentinel;
             const int own_last = nact > 0 ? tlen + 2 * (nact - 1) + 1 : 0;
@@ -461,11 +475,19 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
             const int cap_step = tlen + 2 * cap_lane + cap_half;
             const bool last_pass = p == npass - 1;
 
-            // trace record of (step 1, this lane); one step adds rstep records
-            uint64_t rec = TRACE && active ? trace_rec(qlen, tlen, R, p, 1, gl)
-                                           : 0;
-            const uint32_t rstep =
-                TRACE ? (p + 1 < npass_own ? (uint32_t)L : (uint32_t)nact) : 0u;
+            // trace record of (step 1, this lane); the next step's record
+            // follows it, except after the last step of a segment, which
+            // skips the other lanes' runs of that segment
+            const int t_first = 2 * gl + 1;
+            const int t_last = active ? tlen + 1 + 2 * gl : 0;
+            uint8_t PA_GLOBAL *trp =
+                TRACE && active ? tr + trace_rec(qlen, tlen, R, p, 1, gl) * R
+                                : tr;
+            const uint32_t pass_lanes =
+                TRACE ? (p + 1 < npass_own ? (uint32_t)L : (uint32_t)nact) : 1u;
+            uint32_t seg_skip = (pass_lanes - 1u) * kTraceSeg * R;
+            if (L == 64)   // one task per wave: keep it in a scalar register
+                seg_skip = __builtin_amdgcn_readfirstlane(seg_skip);
             // Column residues and (in later passes) the boundary row reach
             // lane 0 through v_readlane from registers filled 64 columns at a
             // time: lane l holds column base + l, and the next block is
@@ -485,7 +507,7 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
                        (res_at(c + 2) << 16) | (res_at(c + 3) << 24);
             };
             int rc_cur = res_blk(1), rc_nxt = res_blk(65);
-            int bh_cur = 0, bf_cur = NEG, bh_nxt = 0, bf_nxt = NEG;
+            int bh_cur = 0, bf_cur = NEGF, bh_nxt = 0, bf_nxt = NEGF;
             if (p > 0) {
                 // boundary block k holds columns 64k .. 64k+63
                 if (lane <= tlen) {   // 64-lane tasks only
@@ -497,140 +519,162 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
                     bf_nxt = bnd[2 * (64 + lane) + 1];
                 }
             }
-            for (int t = 1; t <= last_step; ++t, rec += rstep) {
-                // column residues: lane 0 takes column t, the others take
-                // lane-1's residue of the previous step (two columns back for
-                // them); no lane-dependent branch surrounds the DPP move
-                {
-                    const int ri = (t - 1) & 63;
-                    if (ri == 0 && t > 1) {
-                        rc_cur = rc_nxt;
-                        rc_nxt = res_blk(t + 64);
-                    }
-                    const int nc = __builtin_amdgcn_update_dpp(
-                        0, c_hi, SHR, 0xf, 0xf, false);
-                    int c0;
-                    if (L == 64) {
-                        c0 = __builtin_amdgcn_readlane(rc_cur, ri);
-                    } else {
-                        const int w = __shfl(rc_cur, grp * L + (ri >> 2));
-                        c0 = (w >> (8 * (ri & 3))) & 0xff;
-                    }
-                    c_hi = c_lo;
-                    c_lo = gl == 0 ? c0 : nc;
-                }
-                int nbh = 0, nbf = NEG;  // boundary for lane 0 at step t+1
-                {
-                    const int jn = t + 1;
-                    if (p == 0) {
-                        nbh = LOCAL ? 0 : open + (jn - 1) * gmax;
-                    } else {
-                        const int bi = jn & 63;
-                        if (bi == 0) {
-                            bh_cur = bh_nxt;
-                            bf_cur = bf_nxt;
-                            const int jb = jn + 64 + lane;
-                            bh_nxt = jb <= tlen ? bnd[2 * jb] : 0;
-                            bf_nxt = jb <= tlen ? bnd[2 * jb + 1] : NEG;
+            // The step loop, specialised for the first pass (boundary row
+            // from the gap model) and later passes (boundary row from
+            // the previous pass).
+            auto run_steps = [&](auto first_pass) {
+                constexpr bool FIRST = decltype(first_pass)::value;
+                for (int t = 1; t <= last_step; ++t) {
+                    // column residues: lane 0 takes column t, the others take
+                    // lane-1's residue of the previous step (two columns back for
+                    // them); no lane-dependent branch surrounds the DPP move
+                    {
+                        const int ri = (t - 1) & 63;
+                        if (ri == 0 && t > 1) {
+                            rc_cur = rc_nxt;
+                            rc_nxt = res_blk(t + 64);
                         }
-                        if (jn <= tlen) {
-                            nbh = __builtin_amdgcn_readlane(bh_cur, bi);
-                            nbf = __builtin_amdgcn_readlane(bf_cur, bi);
-                        }
-                    }
-                }
-                const int jlo = t - 2 * gl;
-                const int jhi = jlo - 1;
-                if (active && jlo >= 1 && jlo <= tlen + 1) {
-                    uint32_t lw[(R + 3) / 4], hw[(R + 3) / 4];
-                    ProfWords<R>::load(prof + c_lo * ROWS + 2 * R * gl, lw);
-                    ProfWords<R>::load(prof + c_hi * ROWS + 2 * R * gl + R,
-                                       hw);
-                    V s[R];
-                    ProfUnpack<O, R, R>::run(lw, hw, s);
-                    V hdiag = dg, hup = upH, fup = upF;
-                    V hoUp = O::add(upH, OPEN);
-                    V cm = O::splat(0);
-                    uint32_t codes[R];
-#pragma unroll
-                    for (int r = 0; r < R; ++r) {
-                        if (!TRACE) {
-                            V e = O::max(O::add(E[r], EXT), Ho[r]);
-                            V f = O::max(O::add(fup, EXT), hoUp);
-                            V hd = O::add(hdiag, s[r]);
-                            V h = O::max(O::max(hd, e), f);
-                            if (LOCAL)
-                                h = O::max(h, O::splat(0));
-                            hdiag = H[r];
-                            H[r] = h;
-                            E[r] = e;
-                            Ho[r] = O::add(h, OPEN);
-                            hoUp = Ho[r];
-                            hup = h;
-                            fup = f;
-                            if (LOCAL)
-                                cm = O::max(cm, h);
+                        int c0;
+                        if (L == 64) {
+                            c0 = __builtin_amdgcn_readlane(rc_cur, ri);
                         } else {
-                            V ep = O::max(O::add(E[r], EXT), Ho[r]);
-                            V e = O::andm(ep, ~3);
-                            V fp = O::max(O::add(fup, EXT), hoUp);
-                            V f = O::andm(fp, ~3);
-                            V hd = O::add(hdiag, s[r]);
-                            V hp = O::max(O::max(hd, O::orm(f, 1)), e);
-                            if (LOCAL)
-                                hp = O::max(hp, O::splat(3));
-                            V h = O::andm(hp, ~3);
-                            codes[r] = O::code(hp, ep, fp);
-                            hdiag = H[r];
-                            H[r] = h;
-                            E[r] = e;
-                            Ho[r] = O::add(h, OPEN);
-                            hoUp = Ho[r];
-                            hup = h;
-                            fup = f;
-                            if (LOCAL)
-                                cm = O::max(cm, h);
+                            const int w = __shfl(rc_cur, grp * L + (ri >> 2));
+                            c0 = (w >> (8 * (ri & 3))) & 0xff;
+                        }
+                        // the first lane of a task has no source lane and keeps
+                        // the DPP "old" operand, column t's residue
+                        const int nc = __builtin_amdgcn_update_dpp(
+                            c0, c_hi, SHR, 0xf, 0xf, false);
+                        c_hi = c_lo;
+                        c_lo = nc;
+                    }
+                    int nbh = 0, nbf = NEGF;  // boundary for lane 0 at step t+1
+                    {
+                        const int jn = t + 1;
+                        if constexpr (FIRST) {
+                            nbh = LOCAL ? 0 : open + (jn - 1) * gmax;
+                        } else {
+                            const int bi = jn & 63;
+                            if (bi == 0) {
+                                bh_cur = bh_nxt;
+                                bf_cur = bf_nxt;
+                                const int jb = jn + 64 + lane;
+                                bh_nxt = jb <= tlen ? bnd[2 * jb] : 0;
+                                bf_nxt = jb <= tlen ? bnd[2 * jb + 1] : NEGF;
+                            }
+                            if (jn <= tlen) {
+                                nbh = __builtin_amdgcn_readlane(bh_cur, bi);
+                                nbf = __builtin_amdgcn_readlane(bf_cur, bi);
+                            }
                         }
                     }
-                    outH = hup;
-                    outF = fup;
-                    if (TRACE) {
-                        typedef typename TraceWord<R>::T W;
-                        W w = TraceWord<R>::pack(codes);
-                        store_word(tr + rec * R, w);
-                    }
-                    if (LOCAL) {
-                        const bool vlo = jlo <= tlen;
-                        const bool vhi = jhi >= 1 && jhi <= tlen;
-                        const Key klo = vlo ? O::key(O::lo(cm) >> SH, jlo) : (Key)0;
-                        const Key khi = vhi ? O::key(O::hi(cm) >> SH, jhi) : (Key)0;
-                        const bool ilo = klo > bk_lo, ihi = khi > bk_hi;
-                        bk_lo = ilo ? klo : bk_lo;
-                        bk_hi = ihi ? khi : bk_hi;
-                        bp_lo = ilo ? p : bp_lo;
-                        bp_hi = ihi ? p : bp_hi;
-                        if (ilo || ihi) {
-#pragma unroll
+                    const int jlo = t - 2 * gl;
+                    const int jhi = jlo - 1;
+                    // columns 1 .. tlen + 1 of the lo half: steps t_first ..
+                    // t_last of this lane
+                    if (t >= t_first && t <= t_last) {
+                        uint32_t lw[(R + 3) / 4], hw[(R + 3) / 4];
+                        ProfWords<R>::load(prof + c_lo * ROWS + 2 * R * gl, lw);
+                        ProfWords<R>::load(prof + c_hi * ROWS + 2 * R * gl + R,
+                                           hw);
+                        V s[R];
+                        O::template unpack<R>(lw, hw, s);
+                        V hdiag = dg, hup = upH, fup = upF;
+                        V hoUp = O::add(upH, OPEN);
+                        V cm = O::splat(0);
+                        V hps[R], efs[R];
+    #pragma unroll
+                        for (int r = 0; r < R; ++r) {
+                            if (!TRACE) {
+                                V e = O::max(O::add(E[r], EXT), Ho[r]);
+                                V f = O::max(O::add(fup, EXT), hoUp);
+                                V hd = O::add(hdiag, s[r]);
+                                V h = O::max(O::max(hd, e), f);
+                                if (LOCAL)
+                                    h = O::max(h, O::splat(0));
+                                hdiag = H[r];
+                                H[r] = h;
+                                E[r] = e;
+                                Ho[r] = O::add(h, OPEN);
+                                hoUp = Ho[r];
+                                hup = h;
+                                fup = f;
+                                if (LOCAL)
+                                    cm = O::max(cm, h);
+                            } else {
+                                // tags: 3 restart or equal-residue
+                                // diagonal, 2 diagonal, 1 F, 0 E
+                                V ep = O::max(O::add(E[r], EXTE), Ho[r]);
+                                V e = O::andm(ep, ~3);
+                                // fup carries tag 01 and hoUp tag 00, so fp
+                                // ends in 01 (extended) or 00 (opened)
+                                V fp = O::max(O::add(fup, EXT), hoUp);
+                                V f1 = O::fmark(fp);
+                                V hd = O::add(hdiag, s[r]);
+                                V hp = O::max(O::max(hd, f1), e);
+                                if (LOCAL)
+                                    hp = O::max(hp, O::splat(3));
+                                V h = O::andm(hp, ~3);
+                                hps[r] = hp;
+                                efs[r] = O::efbits(fp, ep);
+                                hdiag = H[r];
+                                H[r] = h;
+                                E[r] = e;
+                                Ho[r] = O::add(h, OPEN);
+                                hoUp = Ho[r];
+                                hup = h;
+                                fup = f1;
+                                if (LOCAL)
+                                    cm = O::max(cm, h);
+                            }
+                        }
+                        outH = hup;
+                        outF = fup;
+                        if (TRACE) {
+                            uint32_t w[R / 4];
+                            O::template pack<R>(hps, efs, w);
+                            store_rec(trp, w, R / 4);
+                        }
+                        if (LOCAL) {
+                            const bool vlo = jlo <= tlen;
+                            const bool vhi = jhi >= 1 && jhi <= tlen;
+                            const Key klo = vlo ? O::key(O::lo(cm) >> SH, jlo) : (Key)0;
+                            const Key khi = vhi ? O::key(O::hi(cm) >> SH, jhi) : (Key)0;
+                            const bool ilo = klo > bk_lo, ihi = khi > bk_hi;
+                            bk_lo = ilo ? klo : bk_lo;
+                            bk_hi = ihi ? khi : bk_hi;
+                            bp_lo = ilo ? p : bp_lo;
+                            bp_hi = ihi ? p : bp_hi;
+                            if (ilo || ihi) {
+    #pragma unroll
+                                for (int r = 0; r < R; ++r)
+                                    saved[r] = O::blend(saved[r], H[r], ilo, ihi);
+                            }
+                        } else if (last_pass && gl == cap_lane && t == cap_step) {
+    #pragma unroll
                             for (int r = 0; r < R; ++r)
-                                saved[r] = O::blend(saved[r], H[r], ilo, ihi);
+                                if (r == cap_r)
+                                    gscore = cap_half ? O::hi(H[r]) : O::lo(H[r]);
                         }
-                    } else if (last_pass && gl == cap_lane && t == cap_step) {
-#pragma unroll
-                        for (int r = 0; r < R; ++r)
-                            if (r == cap_r)
-                                gscore = cap_half ? O::hi(H[r]) : O::lo(H[r]);
+                        if (p + 1 < npass && gl == L - 1 && jhi >= 0 && jhi <= tlen) {
+                            bnd[2 * jhi] = O::hi(outH);
+                            bnd[2 * jhi + 1] = O::hi(outF);
+                        }
                     }
-                    if (p + 1 < npass && gl == L - 1 && jhi >= 0 && jhi <= tlen) {
-                        bnd[2 * jhi] = O::hi(outH);
-                        bnd[2 * jhi + 1] = O::hi(outF);
+                    dg = upH;
+                    upH = O::template shuffle_up<SHR>(outH, nbh);
+                    upF = O::template shuffle_up<SHR>(outF, nbf);
+                    if (TRACE) {
+                        trp += R;
+                        if (t % (int)kTraceSeg == 0)   // wave-uniform
+                            trp += seg_skip;
                     }
                 }
-                dg = upH;
-                upH = O::template shuffle_up<SHR>(outH);
-                upF = O::template shuffle_up<SHR>(outF);
-                upH = gl == 0 ? O::set_lo(upH, nbh) : upH;
-                upF = gl == 0 ? O::set_lo(upF, nbf) : upF;
-            }
+            };
+            if (p == 0)
+                run_steps(std::true_type{});
+            else
+                run_steps(std::false_type{});
         }
 
         // ---- results ----------------------------------------------------
@@ -706,7 +750,8 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
 namespace pa {
 namespace {
 
-__device__ inline int trace_code(const uint8_t PA_GLOBAL *tr, int R, int qlen,
+template <int R>
+__device__ inline int trace_code(const uint8_t PA_GLOBAL *tr, int qlen,
                                  int tlen, int i, int j)
 {
     const int rows = 128 * R;
@@ -716,11 +761,17 @@ __device__ inline int trace_code(const uint8_t PA_GLOBAL *tr, int R, int qlen,
     const int half = q / R, r = q % R;
     const int t = j + 2 * l + half;
     const uint64_t rec = trace_rec(qlen, tlen, R, p, t, l);
-    const int nib = half * R + r;
-    const uint8_t b = tr[rec * R + (nib >> 1)];
-    return (nib & 1) ? (b >> 4) : (b & 15);
+    // row r, half h of a record: byte 4*(r/4) + 2*(r%2) + h, high nibble
+    // when (r/2) is odd
+    const uint8_t b = tr[rec * R + 4 * (r >> 2) + 2 * (r & 1) + half];
+    return (r & 2) ? (b >> 4) : (b & 15);
 }
 
+// R is fixed per launch (one launch per DP class), so the layout arithmetic
+// divides by constants.  Global traces mark a diagonal step between equal
+// residues with tag 3 (restart does not occur there), so LOCAL = false needs
+// no residue loads.
+template <int R, bool LOCAL>
 __global__ __launch_bounds__(256) void k_traceback(TbArgs a)
 {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -732,18 +783,17 @@ __global__ __launch_bounds__(256) void k_traceback(TbArgs a)
     const uint8_t PA_GLOBAL *qs = glb_addr<const uint8_t>(T.q_off);
     const uint8_t PA_GLOBAL *ts = glb_addr<const uint8_t>(T.t_off);
     const int tlen = (int)T.tlen;
-    const int R = rows_per_lane(T.qlen);
     const uint8_t PA_GLOBAL *tr = glb(a.trace) + glb(a.trace_off)[tid];
     int i = o.q_end, j = o.t_end;
     int matches = 0, cols = 0, runs = 0;
     int last = 0;   // 1 diagonal, 2 vertical, 3 horizontal
     int state = 0;  // 0 H, 1 E (horizontal), 2 F (vertical)
-    if (a.local && o.score <= 0) {
+    if (LOCAL && o.score <= 0) {
         i = j = 0;
     } else {
         for (;;) {
             if (i == 0 || j == 0) {
-                if (a.local)
+                if (LOCAL)
                     break;
                 // global boundary: a single gap run to the origin
                 if (i > 0) {
@@ -757,14 +807,17 @@ __global__ __launch_bounds__(256) void k_traceback(TbArgs a)
                 }
                 break;
             }
-            const int c = trace_code(tr, R, (int)T.qlen, tlen, i, j);
+            const int c = trace_code<R>(tr, (int)T.qlen, tlen, i, j);
             if (state == 0) {
                 const int src = c & 3;
-                if (src == 3)
-                    break;  // restart (local)
-                if (src == 2) {
+                if (LOCAL && src == 3)
+                    break;  // restart
+                if (src >= 2) {
                     ++cols;
-                    matches += qs[i - 1] == ts[j - 1];
+                    if (LOCAL)
+                        matches += qs[i - 1] == ts[j - 1];
+                    else
+                        matches += src == 3;
                     last = 1;
                     --i;
                     --j;
@@ -775,13 +828,13 @@ __global__ __launch_bounds__(256) void k_traceback(TbArgs a)
                 ++cols;
                 runs += last != 3;
                 last = 3;
-                state = (c >> 2) & 1 ? 1 : 0;
+                state = (c >> 3) & 1 ? 1 : 0;   // E extended
                 --j;
             } else {
                 ++cols;
                 runs += last != 2;
                 last = 2;
-                state = (c >> 3) & 1 ? 2 : 0;
+                state = (c >> 2) & 1 ? 2 : 0;   // F extended
                 --i;
             }
         }
@@ -871,9 +924,23 @@ int dp_occupancy(int cls, bool local, bool trace, bool i16)
     return cache[cls][vi];
 }
 
-hipError_t launch_traceback(const TbArgs &a, int grid, hipStream_t s)
+hipError_t launch_traceback(const TbArgs &a, int rows_per_lane, int grid,
+                            hipStream_t s)
 {
-    hipLaunchKernelGGL(k_traceback, dim3(grid), dim3(256), 0, s, a);
+    if (rows_per_lane == 4 && a.local)
+        hipLaunchKernelGGL((k_traceback<4, true>), dim3(grid), dim3(256), 0, s,
+                           a);
+    else if (rows_per_lane == 4)
+        hipLaunchKernelGGL((k_traceback<4, false>), dim3(grid), dim3(256), 0,
+                           s, a);
+    else if (rows_per_lane == 8 && a.local)
+        hipLaunchKernelGGL((k_traceback<8, true>), dim3(grid), dim3(256), 0, s,
+                           a);
+    else if (rows_per_lane == 8)
+        hipLaunchKernelGGL((k_traceback<8, false>), dim3(grid), dim3(256), 0,
+                           s, a);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

@@ -4,6 +4,34 @@
 namespace pa {
 namespace {
 
+// Adds v summed over the block to *dst with one atomic per block (many
+// same-address atomics from every wavefront serialise).  Every thread of the
+// block calls it; blockDim.x is a multiple of 64, at most 1024.
+__device__ inline void block_atomic_add(unsigned long long *dst, uint64_t v)
+{
+    __shared__ uint64_t part[16];
+    for (int off = 32; off >= 1; off >>= 1)
+        v += __shfl_xor(v, off);
+    if ((threadIdx.x & 63) == 0)
+        part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t sum = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w)
+            sum += part[w];
+        if (sum)
+            atomicAdd(dst, (unsigned long long)sum);
+    }
+    __syncthreads();
+}
+
+// grid of a grid-stride loop over n items: at most 2048 blocks of 256
+inline int stride_blocks(uint64_t n)
+{
+    const uint64_t b = (n + 255) / 256;
+    return (int)(b < 2048 ? (b > 0 ? b : 1) : 2048);
+}
+
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, uint32_t n)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -20,9 +48,9 @@ __global__ void k_build_sw_tasks(const pa_pair *__restrict__ pairs, uint64_t n,
                                  DpTask *__restrict__ tasks,
                                  unsigned long long *cells)
 {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     uint64_t c = 0;
-    if (i < n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
         const pa_pair p = pairs[i];
         DpTask t;
         t.q_off = roffs[p.r];
@@ -32,13 +60,9 @@ __global__ void k_build_sw_tasks(const pa_pair *__restrict__ pairs, uint64_t n,
         t.id = (uint32_t)i;
         t.flags = 0;
         tasks[i] = t;
-        c = (uint64_t)t.qlen * t.tlen;
+        c += (uint64_t)t.qlen * t.tlen;
     }
-    // one atomic per wavefront
-    for (int off = 32; off >= 1; off >>= 1)
-        c += __shfl_xor(c, off);
-    if ((threadIdx.x & 63) == 0 && c)
-        atomicAdd(cells, (unsigned long long)c);
+    block_atomic_add(cells, c);
 }
 
 // Pair x seed -> extension task (TSeed(col offset, row offset, k),
@@ -75,9 +99,9 @@ __global__ void k_build_nw_tasks(const XdTask *__restrict__ xt, uint64_t nt,
                                  unsigned long long *cells,
                                  unsigned long long *xcells)
 {
-    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     uint64_t c = 0, xc = 0;
-    if (k < nt) {
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nt;
+         k += (uint64_t)gridDim.x * blockDim.x) {
         const XdTask x = xt[k];
         const XdOut o = xo[x.id];
         DpTask t;
@@ -88,19 +112,11 @@ __global__ void k_build_nw_tasks(const XdTask *__restrict__ xt, uint64_t nt,
         t.id = x.id;
         t.flags = 0;
         tasks[k] = t;
-        c = (uint64_t)t.qlen * t.tlen;
-        xc = (uint64_t)o.cells_left + o.cells_right;
+        c += (uint64_t)t.qlen * t.tlen;
+        xc += (uint64_t)o.cells_left + o.cells_right;
     }
-    for (int off = 32; off >= 1; off >>= 1) {
-        c += __shfl_xor(c, off);
-        xc += __shfl_xor(xc, off);
-    }
-    if ((threadIdx.x & 63) == 0) {
-        if (c)
-            atomicAdd(cells, (unsigned long long)c);
-        if (xc)
-            atomicAdd(xcells, (unsigned long long)xc);
-    }
+    block_atomic_add(cells, c);
+    block_atomic_add(xcells, xc);
 }
 
 __device__ inline bool global_i16_ok(uint32_t q, uint32_t t, int maxs,
@@ -403,7 +419,7 @@ hipError_t build_sw_tasks(const pa_pair *pairs, uint64_t n,
 {
     if (!n)
         return hipSuccess;
-    hipLaunchKernelGGL(k_build_sw_tasks, dim3(blocks(n, 256)), dim3(256), 0, s,
+    hipLaunchKernelGGL(k_build_sw_tasks, dim3(stride_blocks(n)), dim3(256), 0, s,
                        pairs, n, roffs, coffs, tasks, cells);
     return hipGetLastError();
 }
@@ -425,7 +441,7 @@ hipError_t build_nw_tasks(const XdTask *xt, uint64_t nt, const XdOut *xo,
 {
     if (!nt)
         return hipSuccess;
-    hipLaunchKernelGGL(k_build_nw_tasks, dim3(blocks(nt, 256)), dim3(256), 0, s,
+    hipLaunchKernelGGL(k_build_nw_tasks, dim3(stride_blocks(nt)), dim3(256), 0, s,
                        xt, nt, xo, tasks, cells, xcells);
     return hipGetLastError();
 }

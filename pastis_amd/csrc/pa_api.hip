@@ -281,6 +281,8 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
     uint32_t *lists = P<uint32_t>(c->lists);
     uint32_t *sat = P<uint32_t>(c->sat);
     const int dmode = local ? (trace ? 1 : 0) : 2;
+    // tracebacks on the second stream, overlapping later classes' DP
+    const bool split = trace && !c->capturing && c->st2;
     for (auto &ch : chunks) {
         const uint32_t a = (uint32_t)ch.first, b = (uint32_t)ch.second;
         uint64_t base = 0;
@@ -309,6 +311,27 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
         args.bnd_stride = 2u * (c->max_clen + 2);
         args.gap_open = c->cfg.gap_open;
         args.gap_ext = c->cfg.gap_ext;
+        TbArgs tb{};
+        tb.codes = nullptr;
+        tb.tasks = P<DpTask>(c->tasks);
+        tb.trace = trace ? P<uint8_t>(c->trace) - base : nullptr;
+        tb.trace_off = P<uint64_t>(c->toffs);
+        tb.out = P<DpOut>(c->dpout);
+        tb.local = local ? 1 : 0;
+        // traceback of one class's tasks: its two bin lists (bin = 2 * class
+        // + arithmetic)
+        auto traceback = [&](int cls, hipStream_t ts) -> int {
+            Scope sc(c, 3, ts);
+            for (int w = 0; w < 2; ++w) {
+                const int bin = 2 * cls + w;
+                tb.list = lists + (size_t)bin * nt;
+                tb.count = ctr + C_BIN + bin;
+                HIPCHK(c, launch_traceback(tb, cls == 0 || cls == 2 ? 4 : 8,
+                                           (int)((b - a + 255) / 256), ts));
+                c->stats.launches++;
+            }
+            return PA_OK;
+        };
         {
             Scope sc(c, kind, s);
             for (int rcl = 0; rcl < kClasses; ++rcl) {
@@ -347,24 +370,24 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
                                          P<uint8_t>(c->tflags), 1, s));
                     c->stats.launches += 2;
                 }
+                if (split) {
+                    // this class's trace is complete: trace it back on st2
+                    HIPCHK(c, hipEventRecord(c->ev_fork[rcl], s));
+                    HIPCHK(c, hipStreamWaitEvent(c->st2, c->ev_fork[rcl], 0));
+                    if ((rc = traceback(rcl, c->st2)))
+                        return rc;
+                }
             }
         }
-        if (trace) {
-            Scope sc(c, 3, s);
-            TbArgs tb{};
-            tb.codes = nullptr;
-            tb.tasks = P<DpTask>(c->tasks);
-            tb.trace = P<uint8_t>(c->trace) - base;
-            tb.trace_off = P<uint64_t>(c->toffs);
-            tb.out = P<DpOut>(c->dpout);
-            tb.local = local ? 1 : 0;
-            // all non-empty tasks of the chunk: the six bin lists
-            for (int bin = 0; bin < kBins; ++bin) {
-                tb.list = lists + (size_t)bin * nt;
-                tb.count = ctr + C_BIN + bin;
-                HIPCHK(c, launch_traceback(tb, (int)((b - a + 255) / 256), s));
-                c->stats.launches++;
-            }
+        if (split) {
+            // the next chunk reuses the trace store and the bin counters, and
+            // the caller reads the traceback results, all on s
+            HIPCHK(c, hipEventRecord(c->ev_join, c->st2));
+            HIPCHK(c, hipStreamWaitEvent(s, c->ev_join, 0));
+        } else if (trace) {
+            for (int cls = 0; cls < kClasses; ++cls)
+                if ((rc = traceback(cls, s)))
+                    return rc;
         }
     }
     return PA_OK;
@@ -611,6 +634,21 @@ int upload_set(pa_ctx *c, bool rows, const uint8_t *res, const uint64_t *offs,
 }  // namespace
 
 // ---------------------------------------------------------------------------
+namespace {
+void destroy_streams(pa_ctx *c)
+{
+    for (auto &e : c->ev_fork)
+        if (e)
+            hipEventDestroy(e);
+    if (c->ev_join)
+        hipEventDestroy(c->ev_join);
+    if (c->st2)
+        hipStreamDestroy(c->st2);
+    if (c->st)
+        hipStreamDestroy(c->st);
+}
+}  // namespace
+
 extern "C" {
 
 const char *pa_strerror(int status)
@@ -695,8 +733,18 @@ int pa_create(const pa_config *cfg, pa_ctx **out)
         }
     c->cfg.matrix = nullptr;
     c->ncu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
-    if (hipSetDevice(c->device) != hipSuccess ||
-        hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+    bool ok = hipSetDevice(c->device) == hipSuccess &&
+              hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) ==
+                  hipSuccess &&
+              hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking) ==
+                  hipSuccess &&
+              hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) ==
+                  hipSuccess;
+    for (int k = 0; ok && k < kClasses; ++k)
+        ok = hipEventCreateWithFlags(&c->ev_fork[k], hipEventDisableTiming) ==
+             hipSuccess;
+    if (!ok) {
+        destroy_streams(c);
         delete c;
         return PA_ERR_HIP;
     }
@@ -752,8 +800,7 @@ void pa_destroy(pa_ctx *c)
         release(*b);
     for (auto e : c->free_events)
         hipEventDestroy(e);
-    if (c->st)
-        hipStreamDestroy(c->st);
+    destroy_streams(c);
     delete c;
 }
 

@@ -32,6 +32,12 @@ struct pa_ctx {
     int maxs = 0;
     int device = 0;
     hipStream_t st = nullptr;
+    // Tracebacks of traced DP batches run on st2, each DP class's as soon as
+    // that class's DP kernels are done (fork events), while the next class's
+    // DP runs; the join event hands the results back to the batch's stream.
+    hipStream_t st2 = nullptr;
+    hipEvent_t ev_fork[pa::kClasses] = {};
+    hipEvent_t ev_join = nullptr;
     std::string err;
     // sequence sets
     Buf rcodes, ccodes, roffs, coffs;

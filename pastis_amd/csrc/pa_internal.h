@@ -90,14 +90,27 @@ struct TbArgs {
 };
 
 // Trace layout of a task traced with R rows per lane: pass after pass; a
-// pass holds one record of R bytes (2R 4-bit codes) per (step, active lane),
-// step-major.  Full passes have 64 active lanes and tlen + 127 steps; the
-// last pass has nact lanes and tlen + 2*(nact-1) + 1 steps.
+// pass holds one record of R bytes (2R 4-bit codes) per (step, active lane).
+// Full passes have 64 active lanes and tlen + 127 steps; the last pass has
+// nact lanes and tlen + 2*(nact-1) + 1 steps.  Steps are grouped in segments
+// of kTraceSeg: a segment holds, lane after lane, the kTraceSeg consecutive
+// records of each lane, so a traceback that walks back along a lane reads
+// neighbouring bytes (8R contiguous bytes per lane and segment) while one
+// step of the DP still writes 64 records in a few cache lines.  A pass is
+// padded to whole segments.
+constexpr uint32_t kTraceSeg = 8;
+
 __host__ __device__ inline uint32_t trace_last_lanes(uint32_t qlen, int R)
 {
     const uint32_t rows = 128u * (uint32_t)R;
     const uint32_t rem = qlen - ((qlen - 1) / rows) * rows;
     return (rem + 2u * R - 1u) / (2u * R);
+}
+
+// steps of a pass rounded up to whole segments
+__host__ __device__ inline uint32_t trace_seg_steps(uint32_t steps)
+{
+    return (steps + kTraceSeg - 1u) / kTraceSeg * kTraceSeg;
 }
 
 __host__ __device__ inline uint64_t trace_bytes(uint32_t qlen, uint32_t tlen,
@@ -106,8 +119,8 @@ __host__ __device__ inline uint64_t trace_bytes(uint32_t qlen, uint32_t tlen,
     const uint32_t rows = 128u * (uint32_t)R;
     const uint32_t full = (qlen - 1) / rows;
     const uint32_t nl = trace_last_lanes(qlen, R);
-    return ((uint64_t)full * (tlen + 127u) * 64u +
-            (uint64_t)(tlen + 2u * (nl - 1u) + 1u) * nl) *
+    return ((uint64_t)full * trace_seg_steps(tlen + 127u) * 64u +
+            (uint64_t)trace_seg_steps(tlen + 2u * (nl - 1u) + 1u) * nl) *
            (uint64_t)R;
 }
 
@@ -119,7 +132,9 @@ __host__ __device__ inline uint64_t trace_rec(uint32_t qlen, uint32_t tlen,
     const uint32_t rows = 128u * (uint32_t)R;
     const uint32_t full = (qlen - 1) / rows;
     const uint32_t nact = p < full ? 64u : trace_last_lanes(qlen, R);
-    return (uint64_t)p * (tlen + 127u) * 64u + (uint64_t)(t - 1u) * nact + lane;
+    const uint32_t seg = (t - 1u) / kTraceSeg, off = (t - 1u) % kTraceSeg;
+    return (uint64_t)p * trace_seg_steps(tlen + 127u) * 64u +
+           (uint64_t)seg * nact * kTraceSeg + lane * kTraceSeg + off;
 }
 
 // DP kernel class of a task by its row count:
@@ -165,7 +180,9 @@ static_assert(sizeof(XdOut) == 40, "XdOut layout");
 // Launchers (k_dp.hip, k_xdrop.hip, k_misc.hip).
 hipError_t launch_dp(int cls, bool local, bool trace, bool i16,
                      const DpArgs &a, int grid, hipStream_t s);
-hipError_t launch_traceback(const TbArgs &a, int grid, hipStream_t s);
+// Traceback of the tasks of one DP class (rows_per_lane: that class's R).
+hipError_t launch_traceback(const TbArgs &a, int rows_per_lane, int grid,
+                            hipStream_t s);
 // X-drop with one 16-lane row of a wave per work item (k_xdrop_coop.hip);
 // items whose band outgrows 16 columns are appended to ovf_list.
 hipError_t launch_xdrop_coop(const XdTask *tasks, const uint32_t *list,

@@ -123,6 +123,15 @@ hipError_t hipEventCreate(hipEvent_t *e)
     *e = (hipEvent_t)1;
     return hipSuccess;
 }
+hipError_t hipEventCreateWithFlags(hipEvent_t *e, unsigned)
+{
+    return hipEventCreate(e);
+}
+// kernels run to completion at launch, so stream order is program order
+hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned)
+{
+    return hipSuccess;
+}
 hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
 hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
 hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t)

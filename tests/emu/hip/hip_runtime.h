@@ -217,6 +217,9 @@ hipError_t hipStreamSynchronize(hipStream_t);
 hipError_t hipStreamCreateWithFlags(hipStream_t *, unsigned);
 hipError_t hipStreamDestroy(hipStream_t);
 hipError_t hipEventCreate(hipEvent_t *);
+#define hipEventDisableTiming 2u
+hipError_t hipEventCreateWithFlags(hipEvent_t *, unsigned);
+hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned);
 hipError_t hipEventRecord(hipEvent_t, hipStream_t);
 hipError_t hipEventSynchronize(hipEvent_t);
 hipError_t hipEventElapsedTime(float *, hipEvent_t, hipEvent_t);
