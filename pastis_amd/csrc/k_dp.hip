@@ -519,35 +519,54 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
                     bf_nxt = bnd[2 * (64 + lane) + 1];
                 }
             }
+            // Column residues of step tn from those of step tn-1: lane 0
+            // takes column tn, the others take lane-1's hi residue of the
+            // previous step (two columns back for them); no lane-dependent
+            // branch surrounds the DPP move.
+            auto next_res = [&](int tn) {
+                const int ri = (tn - 1) & 63;
+                if (ri == 0 && tn > 1) {
+                    rc_cur = rc_nxt;
+                    rc_nxt = res_blk(tn + 64);
+                }
+                int c0;
+                if (L == 64) {
+                    c0 = __builtin_amdgcn_readlane(rc_cur, ri);
+                } else {
+                    const int w = __shfl(rc_cur, grp * L + (ri >> 2));
+                    c0 = (w >> (8 * (ri & 3))) & 0xff;
+                }
+                // the first lane of a task has no source lane and keeps the
+                // DPP "old" operand, column tn's residue
+                const int nc = __builtin_amdgcn_update_dpp(c0, c_hi, SHR, 0xf,
+                                                           0xf, false);
+                c_hi = c_lo;
+                c_lo = nc;
+            };
+            // residues and profile words of step 1
+            next_res(1);
+            uint32_t lw_n[R / 4], hw_n[R / 4];
+            ProfWords<R>::load(prof + c_lo * ROWS + 2 * R * gl, lw_n);
+            ProfWords<R>::load(prof + c_hi * ROWS + 2 * R * gl + R, hw_n);
             // The step loop, specialised for the first pass (boundary row
             // from the gap model) and later passes (boundary row from
             // the previous pass).
             auto run_steps = [&](auto first_pass) {
                 constexpr bool FIRST = decltype(first_pass)::value;
                 for (int t = 1; t <= last_step; ++t) {
-                    // column residues: lane 0 takes column t, the others take
-                    // lane-1's residue of the previous step (two columns back for
-                    // them); no lane-dependent branch surrounds the DPP move
-                    {
-                        const int ri = (t - 1) & 63;
-                        if (ri == 0 && t > 1) {
-                            rc_cur = rc_nxt;
-                            rc_nxt = res_blk(t + 64);
-                        }
-                        int c0;
-                        if (L == 64) {
-                            c0 = __builtin_amdgcn_readlane(rc_cur, ri);
-                        } else {
-                            const int w = __shfl(rc_cur, grp * L + (ri >> 2));
-                            c0 = (w >> (8 * (ri & 3))) & 0xff;
-                        }
-                        // the first lane of a task has no source lane and keeps
-                        // the DPP "old" operand, column t's residue
-                        const int nc = __builtin_amdgcn_update_dpp(
-                            c0, c_hi, SHR, 0xf, 0xf, false);
-                        c_hi = c_lo;
-                        c_lo = nc;
+                    // profile words of this step (loaded one step ahead);
+                    // then the residues of step t+1 and their profile words,
+                    // so that the LDS latency overlaps this step's work
+                    uint32_t lw[R / 4], hw[R / 4];
+#pragma unroll
+                    for (int k = 0; k < R / 4; ++k) {
+                        lw[k] = lw_n[k];
+                        hw[k] = hw_n[k];
                     }
+                    next_res(t + 1);
+                    ProfWords<R>::load(prof + c_lo * ROWS + 2 * R * gl, lw_n);
+                    ProfWords<R>::load(prof + c_hi * ROWS + 2 * R * gl + R,
+                                       hw_n);
                     int nbh = 0, nbf = NEGF;  // boundary for lane 0 at step t+1
                     {
                         const int jn = t + 1;
@@ -573,10 +592,6 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
                     // columns 1 .. tlen + 1 of the lo half: steps t_first ..
                     // t_last of this lane
                     if (t >= t_first && t <= t_last) {
-                        uint32_t lw[(R + 3) / 4], hw[(R + 3) / 4];
-                        ProfWords<R>::load(prof + c_lo * ROWS + 2 * R * gl, lw);
-                        ProfWords<R>::load(prof + c_hi * ROWS + 2 * R * gl + R,
-                                           hw);
                         V s[R];
                         O::template unpack<R>(lw, hw, s);
                         V hdiag = dg, hup = upH, fup = upF;
