@@ -765,22 +765,6 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
 namespace pa {
 namespace {
 
-template <int R>
-__device__ inline int trace_code(const uint8_t PA_GLOBAL *tr, int qlen,
-                                 int tlen, int i, int j)
-{
-    const int rows = 128 * R;
-    const int i0 = i - 1;
-    const int p = i0 / rows, rr = i0 % rows;
-    const int l = rr / (2 * R), q = rr % (2 * R);
-    const int half = q / R, r = q % R;
-    const int t = j + 2 * l + half;
-    const uint64_t rec = trace_rec(qlen, tlen, R, p, t, l);
-    // row r, half h of a record: byte 4*(r/4) + 2*(r%2) + h, high nibble
-    // when (r/2) is odd
-    const uint8_t b = tr[rec * R + 4 * (r >> 2) + 2 * (r & 1) + half];
-    return (r & 2) ? (b >> 4) : (b & 15);
-}
 
 // R is fixed per launch (one launch per DP class), so the layout arithmetic
 // divides by constants.  Global traces mark a diagonal step between equal
@@ -799,7 +783,44 @@ __global__ __launch_bounds__(256) void k_traceback(TbArgs a)
     const uint8_t PA_GLOBAL *ts = glb_addr<const uint8_t>(T.t_off);
     const int tlen = (int)T.tlen;
     const uint8_t PA_GLOBAL *tr = glb(a.trace) + glb(a.trace_off)[tid];
+    // Position of row i-1 in the trace layout: pass p, lane l, half h, row r
+    // of the half; kept up to date as the walk moves up, so a step costs a
+    // few adds.  Byte offsets within one task's trace fit in 32 bits.
+    constexpr int ROWS = 128 * R;
+    constexpr uint32_t RUN = kTraceSeg * R;   // one lane's run of a segment
+    const uint32_t qlen = T.qlen;
+    const uint32_t full = qlen ? (qlen - 1) / ROWS : 0;
+    const uint32_t last_lanes = qlen ? trace_last_lanes(qlen, R) : 0;
+    const uint32_t pass_bytes = trace_seg_steps((uint32_t)tlen + 127u) * 64u * R;
+    int p = 0, l = 0, h = 0, r = 0;
+    uint32_t pbase = 0, seg_bytes = 0;   // of pass p
+    auto set_pass = [&]() {
+        pbase = (uint32_t)p * pass_bytes;
+        seg_bytes = ((uint32_t)p < full ? 64u : last_lanes) * RUN;
+    };
     int i = o.q_end, j = o.t_end;
+    if (i > 0) {
+        const int i0 = i - 1, rr = i0 % ROWS, q = rr % (2 * R);
+        p = i0 / ROWS;
+        l = rr / (2 * R);
+        h = q / R;
+        r = q % R;
+        set_pass();
+    }
+    // moves the position to row i-2 (the walk's next row)
+    auto up = [&]() {
+        if (--r < 0) {
+            r = R - 1;
+            if (--h < 0) {
+                h = 1;
+                if (--l < 0) {
+                    l = 63;
+                    if (--p >= 0)
+                        set_pass();
+                }
+            }
+        }
+    };
     int matches = 0, cols = 0, runs = 0;
     int last = 0;   // 1 diagonal, 2 vertical, 3 horizontal
     int state = 0;  // 0 H, 1 E (horizontal), 2 F (vertical)
@@ -822,7 +843,14 @@ __global__ __launch_bounds__(256) void k_traceback(TbArgs a)
                 }
                 break;
             }
-            const int c = trace_code<R>(tr, (int)T.qlen, tlen, i, j);
+            // record of (step t, lane l); row r, half h of a record sits in
+            // byte 4*(r/4) + 2*(r%2) + h, high nibble when (r/2) is odd
+            const uint32_t t1 = (uint32_t)(j + 2 * l + h) - 1u;
+            const uint32_t off = pbase + (t1 / kTraceSeg) * seg_bytes +
+                                 (uint32_t)l * RUN + (t1 % kTraceSeg) * R +
+                                 4u * (r >> 2) + 2u * (r & 1) + (uint32_t)h;
+            const uint32_t byte = tr[off];
+            const int c = (int)((r & 2) ? (byte >> 4) : (byte & 15u));
             if (state == 0) {
                 const int src = c & 3;
                 if (LOCAL && src == 3)
@@ -835,6 +863,7 @@ __global__ __launch_bounds__(256) void k_traceback(TbArgs a)
                         matches += src == 3;
                     last = 1;
                     --i;
+                    up();
                     --j;
                 } else {
                     state = src == 1 ? 2 : 1;
@@ -851,6 +880,7 @@ __global__ __launch_bounds__(256) void k_traceback(TbArgs a)
                 last = 2;
                 state = (c >> 2) & 1 ? 2 : 0;   // F extended
                 --i;
+                up();
             }
         }
     }

@@ -21,7 +21,8 @@ hipError_t build_xd_tasks(const pa_pair *pairs, uint64_t n, int seed_cnt,
                           const uint64_t *roffs, const uint64_t *coffs,
                           XdTask *tasks, hipStream_t s);
 hipError_t build_nw_tasks(const XdTask *xt, uint64_t nt, const XdOut *xo,
-                          DpTask *tasks, unsigned long long *cells,
+                          DpTask *tasks, ResBound rows, ResBound cols,
+                          unsigned long long *cells,
                           unsigned long long *xcells, hipStream_t s);
 // Counting sort of tasks [a, b) into per-bin lists: hist and fill are
 // kBins * kBuckets counters that start at zero, counts receives the kBins

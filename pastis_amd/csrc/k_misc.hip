@@ -95,8 +95,8 @@ __global__ void k_build_xd_tasks(const pa_pair *__restrict__ pairs, uint64_t n,
 // (globalAlignment(seqsc_ex, seqsr_ex), SeqanXdropAligner.cpp:284-296).
 __global__ void k_build_nw_tasks(const XdTask *__restrict__ xt, uint64_t nt,
                                  const XdOut *__restrict__ xo,
-                                 DpTask *__restrict__ tasks,
-                                 unsigned long long *cells,
+                                 DpTask *__restrict__ tasks, ResBound rows,
+                                 ResBound cols, unsigned long long *cells,
                                  unsigned long long *xcells)
 {
     uint64_t c = 0, xc = 0;
@@ -111,6 +111,15 @@ __global__ void k_build_nw_tasks(const XdTask *__restrict__ xt, uint64_t nt,
         t.tlen = (uint32_t)(o.end_h - o.beg_h);
         t.id = x.id;
         t.flags = 0;
+        if (t.qlen && t.tlen) {
+            // residue bound of the two spans (traced int16 holds 4*score+3)
+            const uint64_t iq = t.q_off - rows.base, it = t.t_off - cols.base;
+            const uint32_t uq = rows.pre[iq + t.qlen] - rows.pre[iq];
+            const uint32_t ut = cols.pre[it + t.tlen] - cols.pre[it];
+            const uint32_t ub = uq < ut ? uq : ut;
+            if (4ull * ub + 3ull < 32760ull)
+                t.flags |= kTaskBounded;
+        }
         tasks[k] = t;
         c += (uint64_t)t.qlen * t.tlen;
         xc += (uint64_t)o.cells_left + o.cells_right;
@@ -119,15 +128,17 @@ __global__ void k_build_nw_tasks(const XdTask *__restrict__ xt, uint64_t nt,
     block_atomic_add(xcells, xc);
 }
 
-__device__ inline bool global_i16_ok(uint32_t q, uint32_t t, int maxs,
-                                     int open, int ext)
+__device__ inline bool global_i16_ok(uint32_t q, uint32_t t, uint32_t flags,
+                                     int maxs, int open, int ext)
 {
-    // traced int16 kernels hold 4*value + tag in [-32000, 32767]
+    // traced int16 kernels hold 4*value + tag in [-32000, 32767]: the
+    // largest score is at most maxs * min(q, t) and at most the residue
+    // bound (kTaskBounded); the smallest is the all-gap path's
     const int gmax = open > ext ? open : ext;
     const int mn = q < t ? (int)q : (int)t;
     const long ub = (long)maxs * mn;
     const long lb = 2L * open + (long)(q + t) * gmax + (ext < open ? ext : open);
-    return 4 * ub + 3 < 32760 && 4 * lb > -31000;
+    return (4 * ub + 3 < 32760 || (flags & kTaskBounded)) && 4 * lb > -31000;
 }
 
 // Bin and bucket of a task (-1: empty task).  The bucket of a 16-lane class
@@ -139,7 +150,8 @@ __device__ inline int bin_key(const DpTask &t, int local, int maxs, int open,
         return -1;
     const int rc = dp_class(t.qlen);
     const bool i16 =
-        local ? true : global_i16_ok(t.qlen, t.tlen, maxs, open, ext);
+        local ? true
+              : global_i16_ok(t.qlen, t.tlen, t.flags, maxs, open, ext);
     const int bin = 2 * rc + (i16 ? 0 : 1);
     int bucket = 0;
     if (rc <= 1) {
@@ -436,13 +448,14 @@ hipError_t build_xd_tasks(const pa_pair *pairs, uint64_t n, int seed_cnt,
 }
 
 hipError_t build_nw_tasks(const XdTask *xt, uint64_t nt, const XdOut *xo,
-                          DpTask *tasks, unsigned long long *cells,
+                          DpTask *tasks, ResBound rows, ResBound cols,
+                          unsigned long long *cells,
                           unsigned long long *xcells, hipStream_t s)
 {
     if (!nt)
         return hipSuccess;
     hipLaunchKernelGGL(k_build_nw_tasks, dim3(stride_blocks(nt)), dim3(256), 0, s,
-                       xt, nt, xo, tasks, cells, xcells);
+                       xt, nt, xo, tasks, rows, cols, cells, xcells);
     return hipGetLastError();
 }
 

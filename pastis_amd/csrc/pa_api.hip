@@ -545,8 +545,12 @@ int align_xd_dev(pa_ctx *c, const pa_pair *d_pairs, uint64_t n,
     Scope tot(c, 4, s);
     if ((rc = run_xdrop(c, d_pairs, n, s)))
         return rc;
+    const ResBound rb{(uint64_t)(uintptr_t)c->rcodes.p,
+                      (const uint32_t *)((char *)c->rcodes.p + c->rpre_off)};
+    const ResBound cbd{(uint64_t)(uintptr_t)c->ccodes.p,
+                       (const uint32_t *)((char *)c->ccodes.p + c->cpre_off)};
     HIPCHK(c, build_nw_tasks(P<XdTask>(c->xdt), nt, P<XdOut>(c->xdo),
-                             P<DpTask>(c->tasks),
+                             P<DpTask>(c->tasks), rb, cbd,
                              (unsigned long long *)P<uint64_t>(c->cells),
                              (unsigned long long *)(P<uint64_t>(c->cells) + 1),
                              s));
@@ -600,10 +604,28 @@ int upload_set(pa_ctx *c, bool rows, const uint8_t *res, const uint64_t *offs,
                             std::to_string(base + i));
         codes[i] = x;
     }
+    // Residue-bound prefix sums (ResBound) after the codes: the best score of
+    // each code against any code, in either operand position, at least 0.
+    uint32_t best[32] = {};
+    for (int x = 0; x < kAlpha; ++x) {
+        int m = 0;
+        for (int y = 0; y < kAlpha; ++y)
+            m = std::max(m, (int)std::max(c->mat[x * 32 + y], c->mat[y * 32 + x]));
+        best[x] = (uint32_t)m;
+    }
+    const size_t pre_off = (codes.size() + 3) & ~(size_t)3;
+    std::vector<uint32_t> pre(total + 1);
+    uint32_t acc = 0;   // wraps around; span differences stay exact
+    for (uint64_t i = 0; i < total; ++i) {
+        pre[i] = acc;
+        acc += best[codes[i]];
+    }
+    pre[total] = acc;
     Buf &cb = rows ? c->rcodes : c->ccodes;
     Buf &ob = rows ? c->roffs : c->coffs;
-    if ((rc = grow(c, cb, codes.size())))
+    if ((rc = grow(c, cb, pre_off + pre.size() * sizeof(uint32_t))))
         return rc;
+    (rows ? c->rpre_off : c->cpre_off) = pre_off;
     if ((rc = grow(c, ob, (nseq + 1) * sizeof(uint64_t))))
         return rc;
     std::vector<uint64_t> abs(nseq + 1);
@@ -611,6 +633,9 @@ int upload_set(pa_ctx *c, bool rows, const uint8_t *res, const uint64_t *offs,
     for (uint64_t i = 0; i <= nseq; ++i)
         abs[i] = dev + (offs[i] - base);
     HIPCHK(c, hipMemcpyAsync(cb.p, codes.data(), codes.size(),
+                             hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync((char *)cb.p + pre_off, pre.data(),
+                             pre.size() * sizeof(uint32_t),
                              hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipMemcpyAsync(ob.p, abs.data(), abs.size() * 8,
                              hipMemcpyHostToDevice, c->st));
@@ -878,6 +903,7 @@ int pa_load_seqs(pa_ctx *c, int side, const uint8_t *residues,
         c->ccodes = c->rcodes;
         c->coffs = c->roffs;
         c->csk = c->rsk;
+        c->cpre_off = c->rpre_off;
         c->clen = c->rlen;
         c->max_clen = c->max_rlen;
         c->have_cols = true;

@@ -42,6 +42,8 @@ struct pa_ctx {
     // sequence sets
     Buf rcodes, ccodes, roffs, coffs;
     Buf rsk, csk;                 // per-sequence filter sketches
+    // byte offset of the residue-bound prefix sums inside rcodes / ccodes
+    size_t rpre_off = 0, cpre_off = 0;
     bool shared_cols = false;
     std::vector<uint32_t> rlen, clen;
     uint32_t max_rlen = 0, max_clen = 0;

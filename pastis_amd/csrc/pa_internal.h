@@ -56,6 +56,18 @@ static_assert(sizeof(DpOut) == 32, "DpOut layout");
 
 // Task flags
 constexpr uint32_t kTaskSat = 1u;  // int16 saturated, needs the int32 re-run
+// Global task whose scores are bounded by the residue bound below (no cell
+// exceeds min over the two spans of the sum of each residue's best score),
+// small enough for the traced int16 kernels.
+constexpr uint32_t kTaskBounded = 2u;
+
+// Residue bound of a sequence store: pre[k] = sum over codes[0..k) of
+// max(0, best score of that code against any code), as uint32 with
+// wrap-around (differences over one sequence's span stay exact).
+struct ResBound {
+    uint64_t base;          // device address of the store's first residue
+    const uint32_t *pre;    // prefix sums, one more entry than residues
+};
 
 // Arguments of the wavefront DP kernels.
 struct DpArgs {

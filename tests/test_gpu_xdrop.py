@@ -122,3 +122,41 @@ def test_results_do_not_depend_on_tuning():
     al = make_aligner("as_called")
     with pytest.raises(_lib.PaError):
         al.set_tuning(xd_ring=24)
+
+
+def _seed_of(v, h, k=6, band=40):
+    """First shared k-mer near the diagonal (row offset, column offset)."""
+    for i in range(0, len(v) - k):
+        for j in range(max(0, i - band), min(len(h) - k, i + band)):
+            if (v[i:i + k] == h[j:j + k]).all():
+                return i, j
+    return 0, 0
+
+
+@pytest.mark.parametrize("model", ["as_called", "as_documented"])
+def test_sxa_long_homologs(model):
+    """Long infixes: two-pass global DP (over 1024 rows), the int16 kernels
+    admitted by the residue bound (a ~970-residue infix, above the
+    11 * 744 static limit), the int32 kernels (a ~2100-residue infix whose
+    residue bound exceeds int16), and a 300-residue single-pass task."""
+    rng = np.random.default_rng(5)
+    a = rand_seq(rng, 2100)
+    b = a.copy()
+    b[::7] = rand_seq(rng, len(b[::7]))
+    b = np.concatenate([b[:900], rand_seq(rng, 20), b[900:1500], b[1510:]])
+    c = rand_seq(rng, 300)
+    d = c.copy()
+    d[::3] = rand_seq(rng, len(d[::3]))
+    seqs = [a, b, a.copy(), c, d]
+    codes, offs = seqset(seqs)
+    prs = [(0, 1), (0, 2), (3, 4), (1, 0)]
+    seeds = []
+    for r, cc in prs:
+        i, j = _seed_of(seqs[r], seqs[cc])
+        seeds.append([i, j, i, j])
+    pairs = pairs_of(prs, np.array(seeds, np.uint16))
+    print("long homologs", model, flush=True)
+    al = make_aligner(model)
+    al.load_seqs(codes, offs)
+    got = check_align(al, codes, offs, pairs, model)
+    assert got["cols"].max() >= 2000
