@@ -142,8 +142,10 @@ def cpu_baseline_sw(w, pairs, go, ge, budget_s, threads):
 def learned_filter(args, cfg, al, d_pairs, n, plain_out, plain_ms):
     """Train the prefilter on GPU-aligner labels of a separate workload of
     the same generator (X-drop: config 2 with 2M pairs at scale 1, seed 202;
-    SW: config 1, seed 201), calibrate tau for 99.5 % recall on a held-out
-    tenth, then run filter + exact aligner on the benchmark pairs and compare
+    SW: config 1 at twice the scale, seed 201), calibrate tau for 99.5 %
+    recall on a held-out tenth with a one-sided 95 % margin for the sampling
+    error of that tenth (z = 1.645), then run filter + exact aligner on the
+    benchmark pairs and compare
     with the plain operator (pa_align on every pair)."""
     from pastis_amd import prefilter, synth
     from pastis_amd._lib import PAIR_DTYPE, RESULT_DTYPE
@@ -153,7 +155,8 @@ def learned_filter(args, cfg, al, d_pairs, n, plain_out, plain_ms):
         tw = synth.config2(scale=0.4 * args.scale, seed=202,
                            threads=host_cores())
     else:
-        tw = synth.config1(scale=args.scale, seed=201, threads=host_cores())
+        tw = synth.config1(scale=2 * args.scale, seed=201,
+                           threads=host_cores())
     tal = Aligner(cfg)
     tal.load_seqs(tw.codes, tw.offsets)
     y = prefilter.labels_of(tal.align(tw.pairs))
@@ -163,7 +166,7 @@ def learned_filter(args, cfg, al, d_pairs, n, plain_out, plain_ms):
     t_train = time.time()
     losses = flt.train(f[:ntr], y[:ntr], epochs=4, batch=4096, lr=2e-3)
     t_train = time.time() - t_train
-    tau = flt.calibrate(f[ntr:], y[ntr:], recall=0.995)
+    tau = flt.calibrate(f[ntr:], y[ntr:], recall=0.995, z=1.645)
     params, mean, istd = tal.filter_get_model()
     tal.close()
     log("filter: trained on %d pairs (%d positive) in %.2f s, losses %s, "

@@ -161,6 +161,9 @@ __device__ inline int item_step(Item &s, const Rings &A, const ResRings &Rr,
     int32_t adbest = d * gap;
     const int32_t lim = s.best - X;
     const int cols = s.cols, rows = s.rows;
+    // the up value of one cell is the left value of the next (r2 is not
+    // written in this loop)
+    int32_t l = A.at(r2, s.minCol - 1);
     for (int col = s.minCol; col < maxCol0; ++col) {
         int qp, dp;
         if (s.right) {
@@ -170,7 +173,7 @@ __device__ inline int item_step(Item &s, const Rings &A, const ResRings &Rr,
             qp = cols - 1 - col;
             dp = rows - 1 + col - d;
         }
-        const int32_t l = A.at(r2, col - 1), u = A.at(r2, col);
+        const int32_t u = A.at(r2, col);
         int32_t tmp = (l > u ? l : u) + gap;
         const int qr = STAGE ? (int)Rr.qs(col) : (int)s.qs[qp];
         const int dr = STAGE ? (int)Rr.ds(d - col) : (int)s.ds[dp];
@@ -182,6 +185,7 @@ __device__ inline int item_step(Item &s, const Rings &A, const ResRings &Rr,
             A.at(r3, col) = tmp;
             adbest = tmp > adbest ? tmp : adbest;
         }
+        l = u;
     }
     s.nc += (uint32_t)(maxCol0 - s.minCol);
     s.best = adbest > s.best ? adbest : s.best;

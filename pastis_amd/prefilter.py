@@ -41,13 +41,20 @@ def labels_of(results):
     return ((results["flags"] & _lib.PA_FLAG_PASS) != 0).astype(np.uint8)
 
 
-def threshold_for_recall(logits, labels, recall=0.995):
+def threshold_for_recall(logits, labels, recall=0.995, z=0.0):
     """Largest tau whose keep rule (logit >= tau) keeps >= recall of the
-    positives of this calibration set."""
+    positives of this calibration set.
+
+    z > 0 leaves a one-sided margin for the sampling error of the
+    calibration set: of npos positives it allows k = m - z * sqrt(m) misses
+    (m = (1 - recall) * npos, the normal approximation of the binomial
+    count), so that recall on new pairs from the same distribution stays
+    at or above the target with probability about Phi(z)."""
     pos = np.sort(np.asarray(logits, np.float32)[np.asarray(labels) != 0])
     if len(pos) == 0:
         return np.float32(-np.inf)
-    k = int(np.floor((1.0 - recall) * len(pos)))
+    m = (1.0 - recall) * len(pos)
+    k = int(np.floor(max(m - z * np.sqrt(m), 0.0)))
     return np.float32(pos[min(k, len(pos) - 1)])
 
 
@@ -92,8 +99,8 @@ class LearnedFilter:
     def logits(self, feats):
         return self.al.filter_logits(feats)
 
-    def calibrate(self, feats, labels, recall=0.995):
-        self.tau = threshold_for_recall(self.logits(feats), labels, recall)
+    def calibrate(self, feats, labels, recall=0.995, z=0.0):
+        self.tau = threshold_for_recall(self.logits(feats), labels, recall, z)
         return self.tau
 
     def score_pairs(self, pairs):
