@@ -47,6 +47,9 @@ def host_cores():
     return max(1, n)
 
 
+FILTER_FLOP_PER_PAIR = 2 * (16 * 64 + 64 * 64 + 64)
+
+
 def roofline(kstats, st, step_ms_total, tuning=None):
     """Dominant kernel by device time: algorithmic ops / launch time.
 
@@ -180,6 +183,13 @@ def learned_filter(args, cfg, al, d_pairs, n, plain_out, plain_ms):
     z = np.zeros(n, np.float32)
     al.synchronize()
     al.d2h(z, d_log)
+    # inference throughput of the fused features + MLP kernel alone
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        al.filter_score_dev(d_pairs, n, d_log)
+    al.synchronize()
+    score_ms = (time.perf_counter() - t0) * 1e3 / reps
     al.device_free(d_log)
     labels = prefilter.labels_of(plain_out)
     rep = prefilter.report(z, labels, tau)
@@ -221,6 +231,11 @@ def learned_filter(args, cfg, al, d_pairs, n, plain_out, plain_ms):
         "pairs_per_s": round(n / (ms * 1e-3), 1),
         "speedup_vs_plain": round(plain_ms / ms, 3),
         "kept_results_identical_to_plain": exact,
+        # 2 * (16*64 + 64*64 + 64) fp32 FLOP per pair (forward)
+        "score_ms": round(score_ms, 3),
+        "score_pairs_per_s": round(n / (score_ms * 1e-3), 1),
+        "score_tflops": round(n * FILTER_FLOP_PER_PAIR / (score_ms * 1e-3)
+                              / 1e12, 3),
     }
 
 
