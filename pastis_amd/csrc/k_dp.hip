@@ -304,27 +304,12 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
     // query profile of this lane's task
     uint32_t *gprof32 = prof32 + grp * (kProfChars * ROWS / 4);
     const uint8_t *prof = (const uint8_t *)gprof32;
-    // score matrix (score, or 4*score+2 when traced, as int8) after the
-    // profiles; columns >= 25 and rows >= 25 (row 31: padding rows) hold
-    // -128.  Global traces tag a diagonal step between equal residues 3
-    // instead of 2 (4*score+3): the traceback counts identities from the
-    // tags, and diagonal still wins every tie against F and E.
-    uint32_t *mb32 = prof32 + G * kProfChars * ROWS / 4;
-    for (int w = lane; w < 256; w += 64) {
-        uint32_t x = 0;
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const int q = w / 8, c = (w % 8) * 4 + b;
-            int val = -128;
-            if (q < kAlpha && c < kAlpha) {
-                const int sc = a.mat[q * 32 + c];
-                val = TRACE ? 4 * sc + 2 + (!LOCAL && q == c ? 1 : 0) : sc;
-            }
-            x |= ((uint32_t)val & 0xffu) << (8 * b);
-        }
-        mb32[w] = x;
-    }
-    __syncthreads();
+    // profile matrix of this mode in global memory (1 KB, cached): the score,
+    // or 4*score+2 when traced, as int8, -128 outside the alphabet (row 31:
+    // padding rows).  Global traces tag a diagonal step between equal
+    // residues 3 instead of 2 (4*score+3): the traceback counts identities
+    // from the tags, and diagonal still wins every tie against F and E.  The
+    // LDS holds the query profiles only.
     const uint32_t count = *a.count;
     const int open = a.gap_open * (1 << SH);
     const int ext = a.gap_ext * (1 << SH);
@@ -402,7 +387,9 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
                 uint32_t mr[4][8];
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
-                    const uint4 *src = (const uint4 *)(mb32 + qc[k] * 8);
+                    // generic pointer: vector-struct loads from the global
+                    // address space do not compile
+                    const uint4 *src = (const uint4 *)(a.pmat + qc[k] * 8);
                     const uint4 x0 = src[0], x1 = src[1];
                     mr[k][0] = x0.x; mr[k][1] = x0.y; mr[k][2] = x0.z;
                     mr[k][3] = x0.w; mr[k][4] = x1.x; mr[k][5] = x1.y;
@@ -895,8 +882,8 @@ __global__ __launch_bounds__(256) void k_traceback(TbArgs a)
 template <int R, int L, bool LC, bool T, bool I>
 hipError_t launch_one(const DpArgs &a, int grid, hipStream_t s)
 {
-    // G task profiles of 2RL rows each (128R rows in all) + the matrix
-    const size_t lds = (size_t)kProfChars * 128 * R + 1024;
+    // G task profiles of 2RL rows each (128R rows in all)
+    const size_t lds = (size_t)kProfChars * 128 * R;
     hipLaunchKernelGGL((k_dp<R, L, LC, T, I>), dim3(grid), dim3(64), lds, s,
                        a);
     return hipGetLastError();
@@ -905,7 +892,7 @@ hipError_t launch_one(const DpArgs &a, int grid, hipStream_t s)
 template <int R, int L, bool LC, bool T, bool I>
 int occupancy_one()
 {
-    const size_t lds = (size_t)kProfChars * 128 * R + 1024;
+    const size_t lds = (size_t)kProfChars * 128 * R;
     int n = 0;
     hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_dp<R, L, LC, T, I>, 64,
                                                  lds);

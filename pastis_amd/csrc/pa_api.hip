@@ -304,6 +304,10 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
         args.codes = nullptr;  // task offsets hold absolute device addresses
         args.tasks = P<DpTask>(c->tasks);
         args.mat = P<int8_t>(c->dmat);
+        args.pmat = (const uint32_t *)(P<int8_t>(c->dmat) +
+                                       (!local ? kPmatGlobal
+                                               : trace ? kPmatLocal
+                                                       : kPmatUntraced));
         args.out = P<DpOut>(c->dpout);
         args.trace = trace ? P<uint8_t>(c->trace) - base : nullptr;
         args.trace_off = trace ? P<uint64_t>(c->toffs) : nullptr;
@@ -773,8 +777,18 @@ int pa_create(const pa_config *cfg, pa_ctx **out)
         delete c;
         return PA_ERR_HIP;
     }
-    if (grow(c, c->dmat, sizeof(c->mat)) ||
-        hipMemcpy(c->dmat.p, c->mat, sizeof(c->mat), hipMemcpyHostToDevice) !=
+    // the plain table, then the DP kernels' profile matrices (kPmat*)
+    std::vector<int8_t> dm(kPmatBytes, (int8_t)-128);
+    std::memcpy(dm.data(), c->mat, sizeof(c->mat));
+    for (int q = 0; q < kAlpha; ++q)
+        for (int x = 0; x < kAlpha; ++x) {
+            const int sc = c->mat[q * 32 + x];
+            dm[kPmatUntraced + q * 32 + x] = (int8_t)sc;
+            dm[kPmatLocal + q * 32 + x] = (int8_t)(4 * sc + 2);
+            dm[kPmatGlobal + q * 32 + x] = (int8_t)(4 * sc + 2 + (q == x));
+        }
+    if (grow(c, c->dmat, kPmatBytes) ||
+        hipMemcpy(c->dmat.p, dm.data(), kPmatBytes, hipMemcpyHostToDevice) !=
             hipSuccess) {
         pa_destroy(c);
         return PA_ERR_NOMEM;

@@ -56,6 +56,15 @@ static_assert(sizeof(DpOut) == 32, "DpOut layout");
 
 // Task flags
 constexpr uint32_t kTaskSat = 1u;  // int16 saturated, needs the int32 re-run
+
+// Profile matrices: 32x32 int8, row = row-sequence code, column = column
+// code, -128 outside the 25x25 alphabet.  The DP kernels build their query
+// profiles from the one of their mode: the score (untraced), 4*score+2
+// (traced local) or 4*score+2, +1 on the diagonal (traced global: a
+// diagonal step between equal residues carries tag 3).  Byte offsets inside
+// the device copy of the scoring matrix, after the plain 32x32 table.
+constexpr size_t kPmatUntraced = 1024, kPmatLocal = 2048, kPmatGlobal = 3072;
+constexpr size_t kPmatBytes = 4096;
 // Global task whose scores are bounded by the residue bound below (no cell
 // exceeds min over the two spans of the sum of each residue's best score),
 // small enough for the traced int16 kernels.
@@ -77,6 +86,7 @@ struct DpArgs {
     const uint32_t *count;     // device: number of entries in list
     uint32_t *queue;           // device: work-queue head (zeroed before launch)
     const int8_t *mat;         // 32x32 scores
+    const uint32_t *pmat;      // profile matrix of the mode (kPmat* below)
     DpOut *out;
     uint8_t *trace;            // trace store (traced kernels)
     const uint64_t *trace_off; // per task byte offset into trace
