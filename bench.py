@@ -273,7 +273,7 @@ def main():
         return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU "
-              "path (pairs shard by index; see DESIGN.md)", file=sys.stderr)
+              "path (pairs are independent and would shard by similarity-matrix tile, one GPU per tile)", file=sys.stderr)
         sys.exit(2)
 
     from pastis_amd import synth
