@@ -127,6 +127,7 @@ typedef struct pa_stats {
     double ms_trace;         /* device time of the traceback kernels */
     double ms_total;         /* device time of whole calls (event to event) */
     uint64_t launches;       /* kernel launches */
+    uint64_t graph_captures; /* graphs captured by pa_sw_scores_graph */
 } pa_stats;
 
 /* ---- lifetime -------------------------------------------------------- */
@@ -176,7 +177,11 @@ int pa_align(struct pa_ctx *ctx, const pa_pair *pairs, uint64_t n,
 /* The same on device-resident pairs and results (pointers from
  * pa_device_alloc or hipMalloc on the context's device); stream may be NULL
  * for the context's own stream.  Returns after the work is queued; the
- * results are complete when the stream is. */
+ * results are complete when the stream is.  The device entry points do not
+ * return PA_ERR_INDEX / PA_ERR_SEED: a pair naming a sequence outside the
+ * loaded sets, or (X-drop) whose seed does not fit inside its sequences, is
+ * aligned as the empty alignment (score 0, empty extents, not passing), and
+ * no kernel reads outside the sequence store for it. */
 int pa_align_dev(struct pa_ctx *ctx, const pa_pair *d_pairs, uint64_t n,
                  pa_result *d_out, void *stream);
 
@@ -189,9 +194,11 @@ int pa_sw_scores_dev(struct pa_ctx *ctx, const pa_pair *d_pairs, uint64_t n,
 
 /* pa_sw_scores_dev as a replayed HIP graph: the first call with a given
  * (d_pairs, n, d_out, stream) captures the batch's kernels into a graph,
- * later calls replay it.  The pairs' contents may change between calls;
- * their values must stay valid for the loaded sequences (the graph does not
- * re-validate).  A different (d_pairs, n, d_out, stream) recaptures.
+ * later calls replay it.  The pairs' contents may change between calls.  A
+ * different (d_pairs, n, d_out, stream) recaptures, and so does a call after
+ * anything that moved or resized the context's workspace or changed its
+ * sequence sets or tuning (a larger eager batch, pa_load_seqs,
+ * pa_set_tuning), so eager calls may be mixed with replays.
  * Replaces the per-batch launches behind pw_aln_batch's loop
  * (src/DistPWRunner.cpp:116-131) for score-only batches of equal size. */
 int pa_sw_scores_graph(struct pa_ctx *ctx, const pa_pair *d_pairs, uint64_t n,

@@ -16,8 +16,12 @@ extern "C" {
 #endif
 
 /* mode 0 = --sfa (CommonKmerLight tuples), 1 = --sxa (CommonKmerLoc).
- * res / offs: ASCII residues of nseq sequences (one diagonal block).
- * Nonzero i is (rows[i], cols[i]) with count[i], score[i] and, in mode 1,
+ * res / offs: ASCII residues of the nseq row sequences; cres / coffs: the
+ * ncseq column sequences, or NULL for a diagonal block (column set = row
+ * set).  Nonzero i is (rows[i], cols[i]) in block-local indices, aligned
+ * against row sequence rows[i] + bl_roffset and column sequence cols[i] +
+ * bl_coffset (the block offsets pw_aln_batch hands to aln_batch,
+ * src/DistPWRunner.cpp:127-130), with count[i], score[i] and, in mode 1,
  * seeds[4i..4i+3] = first.(row, col), second.(row, col); coordinates must be
  * unique.  ckthr / mosthr: the k-mer prune of DistPWRunner.cpp:48-54;
  * batch_sz: params.aln_batch_sz; cov_thr / ani_thr: params.aln_cov_thr /
@@ -30,7 +34,9 @@ extern "C" {
  * Returns a PA_* status of include/pastis_hip.h. */
 int pw_run_block(int mode, int gap_open, int gap_ext, int klength, int xdrop,
                  int seed_count, int gap_model, int device, const char *res,
-                 const uint64_t *offs, uint64_t nseq, uint64_t nnz,
+                 const uint64_t *offs, uint64_t nseq, const char *cres,
+                 const uint64_t *coffs, uint64_t ncseq, uint64_t bl_roffset,
+                 uint64_t bl_coffset, uint64_t nnz,
                  const uint64_t *rows, const uint64_t *cols,
                  const uint16_t *count, const int16_t *score,
                  const uint16_t *seeds, int ckthr, float mosthr,

@@ -63,7 +63,8 @@ class PaStats(C.Structure):
                 ("xdrop_cells", C.c_uint64), ("rerun_pairs", C.c_uint64),
                 ("ms_sw", C.c_double), ("ms_xdrop", C.c_double),
                 ("ms_nw", C.c_double), ("ms_trace", C.c_double),
-                ("ms_total", C.c_double), ("launches", C.c_uint64)]
+                ("ms_total", C.c_double), ("launches", C.c_uint64),
+                ("graph_captures", C.c_uint64)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

@@ -13,12 +13,16 @@ constexpr int kBins = 2 * kClasses;
 constexpr int kBuckets = 16;
 
 hipError_t fill_u32(uint32_t *p, uint32_t v, uint32_t n, hipStream_t s);
+// Pairs naming sequences outside the nr row / nc column sequences become
+// empty tasks (and X-drop tasks whose seed does not fit are flagged
+// kXdInvalid), so no kernel reads outside the sequence store.
 hipError_t build_sw_tasks(const pa_pair *pairs, uint64_t n,
                           const uint64_t *roffs, const uint64_t *coffs,
-                          DpTask *tasks, unsigned long long *cells,
-                          hipStream_t s);
+                          uint64_t nr, uint64_t nc, DpTask *tasks,
+                          unsigned long long *cells, hipStream_t s);
 hipError_t build_xd_tasks(const pa_pair *pairs, uint64_t n, int seed_cnt,
-                          const uint64_t *roffs, const uint64_t *coffs,
+                          int seed_len, const uint64_t *roffs,
+                          const uint64_t *coffs, uint64_t nr, uint64_t nc,
                           XdTask *tasks, hipStream_t s);
 hipError_t build_nw_tasks(const XdTask *xt, uint64_t nt, const XdOut *xo,
                           DpTask *tasks, ResBound rows, ResBound cols,

@@ -42,9 +42,13 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t v, uint32_t n)
 // Pair -> local-alignment task: rows = row sequence (SeqAn V), columns =
 // column sequence (SeqAn H), as localAlignment(seqsc, seqsr) at
 // SeqanFullAligner.cpp:231.
+// A pair naming a sequence outside the loaded sets (nr rows, nc columns)
+// becomes an empty task: its result is the empty alignment and no kernel
+// reads residues for it.
 __global__ void k_build_sw_tasks(const pa_pair *__restrict__ pairs, uint64_t n,
                                  const uint64_t *__restrict__ roffs,
                                  const uint64_t *__restrict__ coffs,
+                                 uint64_t nr, uint64_t nc,
                                  DpTask *__restrict__ tasks,
                                  unsigned long long *cells)
 {
@@ -52,11 +56,13 @@ __global__ void k_build_sw_tasks(const pa_pair *__restrict__ pairs, uint64_t n,
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (uint64_t)gridDim.x * blockDim.x) {
         const pa_pair p = pairs[i];
+        const bool ok = p.r < nr && p.c < nc;
+        const uint32_t r = ok ? p.r : 0, cc = ok ? p.c : 0;
         DpTask t;
-        t.q_off = roffs[p.r];
-        t.qlen = (uint32_t)(roffs[p.r + 1] - roffs[p.r]);
-        t.t_off = coffs[p.c];
-        t.tlen = (uint32_t)(coffs[p.c + 1] - coffs[p.c]);
+        t.q_off = roffs[r];
+        t.qlen = ok ? (uint32_t)(roffs[r + 1] - roffs[r]) : 0u;
+        t.t_off = coffs[cc];
+        t.tlen = ok ? (uint32_t)(coffs[cc + 1] - coffs[cc]) : 0u;
         t.id = (uint32_t)i;
         t.flags = 0;
         tasks[i] = t;
@@ -66,11 +72,15 @@ __global__ void k_build_sw_tasks(const pa_pair *__restrict__ pairs, uint64_t n,
 }
 
 // Pair x seed -> extension task (TSeed(col offset, row offset, k),
-// SeqanXdropAligner.cpp:273-280).
+// SeqanXdropAligner.cpp:273-280).  A pair naming a sequence outside the
+// loaded sets, or a seed that does not fit inside its sequences, gets
+// kXdInvalid: the extension kernels then write empty extents without reading
+// residues, so its infix alignment is empty and it does not pass.
 __global__ void k_build_xd_tasks(const pa_pair *__restrict__ pairs, uint64_t n,
-                                 int seed_cnt,
+                                 int seed_cnt, int seed_len,
                                  const uint64_t *__restrict__ roffs,
                                  const uint64_t *__restrict__ coffs,
+                                 uint64_t nr, uint64_t nc,
                                  XdTask *__restrict__ tasks)
 {
     const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -79,15 +89,19 @@ __global__ void k_build_xd_tasks(const pa_pair *__restrict__ pairs, uint64_t n,
     const uint64_t i = k / seed_cnt;
     const int s = (int)(k % seed_cnt);
     const pa_pair p = pairs[i];
+    const bool ok = p.r < nr && p.c < nc;
+    const uint32_t r = ok ? p.r : 0, c = ok ? p.c : 0;
     XdTask t;
-    t.v_off = roffs[p.r];
-    t.nv = (uint32_t)(roffs[p.r + 1] - roffs[p.r]);
-    t.h_off = coffs[p.c];
-    t.nh = (uint32_t)(coffs[p.c + 1] - coffs[p.c]);
+    t.v_off = roffs[r];
+    t.nv = ok ? (uint32_t)(roffs[r + 1] - roffs[r]) : 0u;
+    t.h_off = coffs[c];
+    t.nh = ok ? (uint32_t)(coffs[c + 1] - coffs[c]) : 0u;
     t.beg_v = s == 0 ? p.seed[0] : p.seed[2];
     t.beg_h = s == 0 ? p.seed[1] : p.seed[3];
     t.id = (uint32_t)(i * 2 + s);
-    t.pad = 0;
+    const bool fits = ok && t.beg_v + (uint32_t)seed_len <= t.nv &&
+                      t.beg_h + (uint32_t)seed_len <= t.nh;
+    t.pad = fits ? 0u : kXdInvalid;
     tasks[k] = t;
 }
 
@@ -426,24 +440,26 @@ hipError_t fill_u32(uint32_t *p, uint32_t v, uint32_t n, hipStream_t s)
 
 hipError_t build_sw_tasks(const pa_pair *pairs, uint64_t n,
                           const uint64_t *roffs, const uint64_t *coffs,
-                          DpTask *tasks, unsigned long long *cells,
-                          hipStream_t s)
+                          uint64_t nr, uint64_t nc, DpTask *tasks,
+                          unsigned long long *cells, hipStream_t s)
 {
     if (!n)
         return hipSuccess;
     hipLaunchKernelGGL(k_build_sw_tasks, dim3(stride_blocks(n)), dim3(256), 0, s,
-                       pairs, n, roffs, coffs, tasks, cells);
+                       pairs, n, roffs, coffs, nr, nc, tasks, cells);
     return hipGetLastError();
 }
 
 hipError_t build_xd_tasks(const pa_pair *pairs, uint64_t n, int seed_cnt,
-                          const uint64_t *roffs, const uint64_t *coffs,
+                          int seed_len, const uint64_t *roffs,
+                          const uint64_t *coffs, uint64_t nr, uint64_t nc,
                           XdTask *tasks, hipStream_t s)
 {
     if (!n)
         return hipSuccess;
     hipLaunchKernelGGL(k_build_xd_tasks, dim3(blocks(n * seed_cnt, 256)),
-                       dim3(256), 0, s, pairs, n, seed_cnt, roffs, coffs, tasks);
+                       dim3(256), 0, s, pairs, n, seed_cnt, seed_len, roffs,
+                       coffs, nr, nc, tasks);
     return hipGetLastError();
 }
 

@@ -262,6 +262,20 @@ __device__ inline void item_write(const XdArgs &a, uint32_t k, int32_t score,
 {
     const XdTask T = a.tasks[k >> 1];
     XdOut PA_GLOBAL &o = glb(a.out)[T.id];
+    if (T.pad & kXdInvalid) {   // empty extents [0, 0) on both sequences
+        if ((k & 1) == 0) {
+            o.beg_h = o.beg_v = 0;
+            o.score_left = 0;
+            o.cells_left = 0;
+            o.flags_left = 0;
+        } else {
+            o.end_h = o.end_v = 0;
+            o.score_right = 0;
+            o.cells_right = 0;
+            o.flags_right = 0;
+        }
+        return;
+    }
     const int beg_h = (int)T.beg_h, beg_v = (int)T.beg_v;
     if ((k & 1) == 0) {
         o.beg_h = upd ? beg_h - row : beg_h;
@@ -356,7 +370,7 @@ __global__ __launch_bounds__(RING ? 64 : 256) void k_xdrop_pl(XdArgs a)
                         s.db = s.ds - 1;
                         s.dstep = 1;
                     }
-                    if (s.rows == 1 || s.cols == 1) {
+                    if ((T.pad & kXdInvalid) || s.rows <= 1 || s.cols <= 1) {
                         item_write(a, k, 0, 0, 0, false, 0, 0);
                     } else {
                         item_init<ST>(s, A, Rr, gap, X);

@@ -97,6 +97,7 @@ __global__ __launch_bounds__(256) void k_xdrop_coop(CoopArgs a)
     int cols = 0, rows = 0, d = 0, minCol = 0, maxCol = 0, best = 0;
     int off1 = 0, off2 = 0, off3 = 0, l1 = 0, l2 = 0, l3 = 0;
     bool right = false;
+    bool invalid = false;   // kXdInvalid task: empty extents
     uint32_t nc = 0;
     // this lane's values on anti-diagonals d-2, d-1, d
     int A1 = kUndef, A2 = kUndef, A3 = kUndef;
@@ -159,8 +160,9 @@ __global__ __launch_bounds__(256) void k_xdrop_coop(CoopArgs a)
                         best = 0;
                         nc = 0;
                         state = kRun;
-                        if (rows == 1 || cols == 1)
+                        if ((T.pad & kXdInvalid) || rows <= 1 || cols <= 1)
                             state = 3;   // finished without an update
+                        invalid = (T.pad & kXdInvalid) != 0;
                     }
                 }
                 cbase += take;
@@ -291,14 +293,18 @@ __global__ __launch_bounds__(256) void k_xdrop_coop(CoopArgs a)
                     a.ovf_list[pos] = k;
                 }
                 if (!right) {
-                    o.beg_h = upd ? beg_h - lrow : beg_h;
-                    o.beg_v = upd ? beg_v - lcol : beg_v;
+                    o.beg_h = invalid ? 0 : (upd ? beg_h - lrow : beg_h);
+                    o.beg_v = invalid ? 0 : (upd ? beg_v - lcol : beg_v);
                     o.score_left = state == 3 ? 0 : ls;
                     o.cells_left = nc;
                     o.flags_left = fl;
                 } else {
-                    o.end_h = upd ? beg_h + a.seed_len + lrow : beg_h + a.seed_len;
-                    o.end_v = upd ? beg_v + a.seed_len + lcol : beg_v + a.seed_len;
+                    o.end_h = invalid ? 0
+                              : (upd ? beg_h + a.seed_len + lrow
+                                     : beg_h + a.seed_len);
+                    o.end_v = invalid ? 0
+                              : (upd ? beg_v + a.seed_len + lcol
+                                     : beg_v + a.seed_len);
                     o.score_right = state == 3 ? 0 : ls;
                     o.cells_right = nc;
                     o.flags_right = fl;

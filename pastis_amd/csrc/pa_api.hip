@@ -444,7 +444,8 @@ int align_sw_dev(pa_ctx *c, const pa_pair *d_pairs, uint64_t n,
         return rc;
     Scope tot(c, 4, s);
     HIPCHK(c, build_sw_tasks(d_pairs, n, P<uint64_t>(c->roffs),
-                             P<uint64_t>(c->coffs), P<DpTask>(c->tasks),
+                             P<uint64_t>(c->coffs), c->rlen.size(),
+                             c->clen.size(), P<DpTask>(c->tasks),
                              (unsigned long long *)P<uint64_t>(c->cells), s));
     if ((rc = run_dp(c, n, true, true, 0, s)))
         return rc;
@@ -471,8 +472,10 @@ int run_xdrop(pa_ctx *c, const pa_pair *d_pairs, uint64_t n, hipStream_t s)
     uint32_t *ovf = P<uint32_t>(c->ovf), *lng = ovf + ni;
     HIPCHK(c, fill_u32(ctr + C_XDQ, 0, C_N - C_XDQ, s));
     HIPCHK(c, fill_u32(ctr + C_XDN, (uint32_t)ni, 1, s));
-    HIPCHK(c, build_xd_tasks(d_pairs, n, sc, P<uint64_t>(c->roffs),
-                             P<uint64_t>(c->coffs), P<XdTask>(c->xdt), s));
+    HIPCHK(c, build_xd_tasks(d_pairs, n, sc, c->cfg.seed_len,
+                             P<uint64_t>(c->roffs), P<uint64_t>(c->coffs),
+                             c->rlen.size(), c->clen.size(),
+                             P<XdTask>(c->xdt), s));
     {
         Scope t(c, 1, s);
         const int grid = xd_grid(c, ni);
@@ -862,6 +865,7 @@ int pa_set_tuning(pa_ctx *c, const pa_tuning *t)
     if (t->xd_coop != 0 && t->xd_coop != 1)
         return fail(c, PA_ERR_ARG, "xd_coop must be 0 or 1");
     c->tune = *t;
+    ++c->gen;   // grids of captured graphs follow the tuning
     return PA_OK;
 }
 
@@ -885,6 +889,8 @@ int pa_load_seqs(pa_ctx *c, int side, const uint8_t *residues,
         (encoding != PA_ENC_CODES && encoding != PA_ENC_ASCII))
         return fail(c, PA_ERR_ARG, "bad side or encoding");
     hipSetDevice(c->device);
+    // captured graphs hold the set sizes and the boundary-row stride
+    ++c->gen;
     int rc;
     if (side == PA_SIDE_ROW || side == PA_SIDE_BOTH) {
         if (c->shared_cols) {
@@ -988,7 +994,8 @@ int pa_sw_scores_dev(pa_ctx *c, const pa_pair *d_pairs, uint64_t n,
         return rc;
     Scope tot(c, 4, s);
     HIPCHK(c, build_sw_tasks(d_pairs, n, P<uint64_t>(c->roffs),
-                             P<uint64_t>(c->coffs), P<DpTask>(c->tasks),
+                             P<uint64_t>(c->coffs), c->rlen.size(),
+                             c->clen.size(), P<DpTask>(c->tasks),
                              (unsigned long long *)P<uint64_t>(c->cells), s));
     if ((rc = run_dp(c, n, true, false, 0, s)))
         return rc;
@@ -1012,7 +1019,7 @@ int pa_sw_scores_graph(pa_ctx *c, const pa_pair *d_pairs, uint64_t n,
     hipSetDevice(c->device);
     hipStream_t s = pick(c, stream);
     if (!c->gexec || c->g_pairs != d_pairs || c->g_n != n || c->g_out != d_out ||
-        c->g_stream != s) {
+        c->g_stream != s || c->g_gen != c->gen) {
         if (c->gexec)
             hipGraphExecDestroy(c->gexec);
         if (c->graph)
@@ -1026,7 +1033,8 @@ int pa_sw_scores_graph(pa_ctx *c, const pa_pair *d_pairs, uint64_t n,
         HIPCHK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         c->capturing = true;
         rc = build_sw_tasks(d_pairs, n, P<uint64_t>(c->roffs),
-                            P<uint64_t>(c->coffs), P<DpTask>(c->tasks),
+                            P<uint64_t>(c->coffs), c->rlen.size(),
+                            c->clen.size(), P<DpTask>(c->tasks),
                             (unsigned long long *)P<uint64_t>(c->cells), s) !=
                      hipSuccess
                  ? PA_ERR_HIP
@@ -1049,6 +1057,8 @@ int pa_sw_scores_graph(pa_ctx *c, const pa_pair *d_pairs, uint64_t n,
         c->g_n = n;
         c->g_out = d_out;
         c->g_stream = s;
+        c->g_gen = c->gen;
+        c->stats.graph_captures++;
     }
     HIPCHK(c, hipGraphLaunch(c->gexec, s));
     c->stats.pairs += n;

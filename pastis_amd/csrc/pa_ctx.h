@@ -70,6 +70,10 @@ struct pa_ctx {
     const void *g_pairs = nullptr, *g_out = nullptr;
     uint64_t g_n = 0;
     hipStream_t g_stream = nullptr;
+    // Workspace generation: bumped whenever a buffer the graph may have
+    // captured is reallocated (grow), the sequence sets change or the launch
+    // shapes change; a graph captured at another generation is recaptured.
+    uint64_t gen = 0, g_gen = 0;
     pa_tuning tune{0, 0, 16, 0};
 };
 
@@ -94,6 +98,8 @@ inline int grow(pa_ctx *c, Buf &b, size_t bytes)
 {
     if (b.cap >= bytes && b.p)
         return PA_OK;
+    if (c)
+        ++c->gen;   // captured graphs hold the old address
     if (b.p)
         hipFree(b.p);
     b.p = nullptr;

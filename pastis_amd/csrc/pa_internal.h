@@ -184,9 +184,11 @@ struct XdTask {
     uint32_t nh, nv;           // lengths
     uint32_t beg_h, beg_v;     // seed start
     uint32_t id;               // output slot
-    uint32_t pad;
+    uint32_t pad;              // kXdInvalid: no extension, empty extents
 };
 static_assert(sizeof(XdTask) == 40, "XdTask layout");
+// The pair names a sequence that is not loaded or its seed does not fit.
+constexpr uint32_t kXdInvalid = 1u;
 
 // Seed extension result; the left direction writes beg_*, score_left,
 // cells_left, flags_left and the right direction the other half.

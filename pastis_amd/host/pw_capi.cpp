@@ -16,10 +16,11 @@ using namespace pastis_amd;
 namespace {
 
 template <class NT>
-int run_block(HipAligner &al, const params_t &prm, uint64_t nnz,
-              const uint64_t *rows, const uint64_t *cols,
-              const uint16_t *count, const int16_t *score,
-              const uint16_t *seeds, float *score_aln_out, uint64_t *stats)
+int run_block(HipAligner &al, const params_t &prm, uint64_t bl_roffset,
+              uint64_t bl_coffset, uint64_t nnz, const uint64_t *rows,
+              const uint64_t *cols, const uint16_t *count,
+              const int16_t *score, const uint16_t *seeds,
+              float *score_aln_out, uint64_t *stats)
 {
     // DCSC of the nonzeros, columns ascending, rows ascending in a column
     std::vector<uint64_t> ord(nnz);
@@ -54,7 +55,7 @@ int run_block(HipAligner &al, const params_t &prm, uint64_t nnz,
         where[(rows[i] << 32) | cols[i]] = i;
     if (where.size() != nnz)
         return PA_ERR_ARG;   // duplicate coordinates
-    RunnerStats st = pw_aln_batch(C, al, prm);
+    RunnerStats st = pw_aln_batch(C, al, prm, bl_roffset, bl_coffset);
     std::fill(score_aln_out, score_aln_out + nnz, 0.0f);
     for (uint64_t i = 0; i < C.nzc(); ++i)
         for (uint64_t j = C.cp[i]; j < C.cp[i + 1]; ++j)
@@ -73,16 +74,22 @@ int run_block(HipAligner &al, const params_t &prm, uint64_t nnz,
 
 extern "C" {
 
-// mode 0: --sfa (Light tuples), 1: --sxa (Loc tuples).  Sequences are one
-// diagonal block of ASCII residues.  Nonzero i is (rows[i], cols[i]) with
-// count, score and, for --sxa, seeds[4*i .. 4*i+3] = first.(row, col),
-// second.(row, col).  score_aln_out[i] receives the final value (0 when the
-// nonzero was pruned).  filter_params == NULL runs without the learned
-// filter.  stats (8 values, may be NULL): nnz in, after the k-mer prune,
-// after the alignment prune, batches, aligned, filtered out, passed, pairs.
+// mode 0: --sfa (Light tuples), 1: --sxa (Loc tuples).  res / offs hold
+// the row sequence set as ASCII residues; cres / coffs the column set, or
+// NULL for a diagonal block (the column set is the row set).  Nonzero i is
+// (rows[i], cols[i]) in block-local indices, aligned against row sequence
+// rows[i] + bl_roffset and column sequence cols[i] + bl_coffset (the block
+// offsets of DistPWRunner.cpp:127-130), with count, score and, for --sxa,
+// seeds[4*i .. 4*i+3] = first.(row, col), second.(row, col).
+// score_aln_out[i] receives the final value (0 when the nonzero was
+// pruned).  filter_params == NULL runs without the learned filter.  stats
+// (8 values, may be NULL): nnz in, after the k-mer prune, after the
+// alignment prune, batches, aligned, filtered out, passed, pairs.
 int pw_run_block(int mode, int gap_open, int gap_ext, int klength, int xdrop,
                  int seed_count, int gap_model, int device, const char *res,
-                 const uint64_t *offs, uint64_t nseq, uint64_t nnz,
+                 const uint64_t *offs, uint64_t nseq, const char *cres,
+                 const uint64_t *coffs, uint64_t ncseq, uint64_t bl_roffset,
+                 uint64_t bl_coffset, uint64_t nnz,
                  const uint64_t *rows, const uint64_t *cols,
                  const uint16_t *count, const int16_t *score,
                  const uint16_t *seeds, int ckthr, float mosthr,
@@ -93,7 +100,8 @@ int pw_run_block(int mode, int gap_open, int gap_ext, int klength, int xdrop,
 {
     if (!offs || (nnz && (!rows || !cols || !count || !score ||
                           !score_aln_out)) ||
-        (mode == 1 && nnz && !seeds) || batch_sz == 0)
+        (mode == 1 && nnz && !seeds) || batch_sz == 0 ||
+        (cres != nullptr) != (coffs != nullptr))
         return PA_ERR_ARG;
     try {
         params_t prm;
@@ -117,17 +125,25 @@ int pw_run_block(int mode, int gap_open, int gap_ext, int klength, int xdrop,
                                     xdrop, seed_count, device, gap_model));
         std::vector<char> r(res, res + offs[nseq]);
         std::vector<uint64_t> o(offs, offs + nseq + 1);
-        al->construct_seqs_bl(
-            std::make_shared<VectorSource>(std::move(r), std::move(o)));
+        if (coffs) {
+            std::vector<char> cr(cres, cres + coffs[ncseq]);
+            std::vector<uint64_t> co(coffs, coffs + ncseq + 1);
+            al->construct_seqs_bl(std::make_shared<VectorSource>(
+                std::move(r), std::move(o), std::move(cr), std::move(co)));
+        } else {
+            al->construct_seqs_bl(
+                std::make_shared<VectorSource>(std::move(r), std::move(o)));
+        }
         if (filter_params)
             al->set_filter(filter_params, filter_mean, filter_istd,
                            filter_tau);
         if (mode == 0)
-            return run_block<CommonKmerLight>(*al, prm, nnz, rows, cols, count,
-                                              score, seeds, score_aln_out,
-                                              stats);
-        return run_block<CommonKmerLoc>(*al, prm, nnz, rows, cols, count,
-                                        score, seeds, score_aln_out, stats);
+            return run_block<CommonKmerLight>(*al, prm, bl_roffset, bl_coffset,
+                                              nnz, rows, cols, count, score,
+                                              seeds, score_aln_out, stats);
+        return run_block<CommonKmerLoc>(*al, prm, bl_roffset, bl_coffset, nnz,
+                                        rows, cols, count, score, seeds,
+                                        score_aln_out, stats);
     } catch (const std::exception &e) {
         std::fprintf(stderr, "pw_run_block: %s\n", e.what());
         return PA_ERR_HIP;

@@ -27,7 +27,8 @@ def lib():
         _lib.lib()   # libpastis_hip.so first (same directory, rpath)
         L = C.CDLL(PW_PATH)
         vp, u64, i32 = C.c_void_p, C.c_uint64, C.c_int
-        L.pw_run_block.argtypes = ([i32] * 8 + [vp, vp, u64, u64] + [vp] * 5 +
+        L.pw_run_block.argtypes = ([i32] * 8 + [vp, vp, u64, vp, vp, u64,
+                                                u64, u64, u64] + [vp] * 5 +
                                    [i32, C.c_float, u64, C.c_double, i32,
                                     vp, vp, vp, C.c_float, vp, vp])
         _pw = L
@@ -46,11 +47,18 @@ def _p(a):
 def run_block(mode, codes, offsets, rows, cols, count, score, seeds=None,
               gap_open=-11, gap_ext=-2, klength=6, xdrop=49, seed_count=2,
               gap_model="as_called", ckthr=0, mosthr=-1.0, batch_sz=10 ** 10,
-              cov_thr=0.7, ani_thr=30, filt=None, device=0):
+              cov_thr=0.7, ani_thr=30, filt=None, device=0, ccodes=None,
+              coffsets=None, bl_roffset=0, bl_coffset=0):
     """Runs one block; returns (score_aln per nonzero, stats dict).
-    filt = (params, mean, istd, tau) enables the learned prefilter."""
+    filt = (params, mean, istd, tau) enables the learned prefilter.
+    ccodes / coffsets give a separate column set (default: the row set);
+    rows / cols are block-local, offset by bl_roffset / bl_coffset."""
     res = np.ascontiguousarray(ascii_of(codes))
     offs = np.ascontiguousarray(offsets, np.uint64)
+    cres = cof = None
+    if ccodes is not None:
+        cres = np.ascontiguousarray(ascii_of(ccodes))
+        cof = np.ascontiguousarray(coffsets, np.uint64)
     rows = np.ascontiguousarray(rows, np.uint64)
     cols = np.ascontiguousarray(cols, np.uint64)
     count = np.ascontiguousarray(count, np.uint16)
@@ -71,7 +79,9 @@ def run_block(mode, codes, offsets, rows, cols, count, score, seeds=None,
         tau = float(filt[3])
     rc = lib().pw_run_block(mode, gap_open, gap_ext, klength, xdrop,
                             seed_count, gm, device, _p(res), _p(offs),
-                            len(offs) - 1, n, _p(rows), _p(cols), _p(count),
+                            len(offs) - 1, _p(cres), _p(cof),
+                            0 if cof is None else len(cof) - 1, bl_roffset,
+                            bl_coffset, n, _p(rows), _p(cols), _p(count),
                             _p(score), _p(seeds), ckthr, mosthr, batch_sz,
                             cov_thr, ani_thr, _p(fp), _p(fm), _p(fs), tau,
                             _p(out), _p(st))

@@ -141,3 +141,97 @@ def local_optimal_paths(h, v, mat, go, ge):
                 if s == top and s > 0:
                     res.append((i0, j0, i1, j1, mv))
     return top, res
+
+
+# ---------------------------------------------------------------------------
+# The traceback policy, restated over explicit paths.
+#
+# An annotated path spells every gap move as opening a DP gap run ("o") or
+# extending one ("e"); an extension must follow a move of the same
+# direction.  Scores: D adds the substitution score, an opening move adds go,
+# an extending move adds ge.  The best annotated score of a move string is
+# path_score's.  The policy the oracle and the kernels document --
+# restart > diagonal > vertical > horizontal in the H state, and a tie
+# between extending and opening a gap run keeps the extension -- selects,
+# among the optimal annotated paths ending at the reported end cell, the one
+# that is lexicographically greatest when read backward from the end under
+# the order  stop > D > Ve > Vo > He > Ho  ("stop": the path begins here).
+# This picks the path by comparing whole paths; it does not walk a DP.
+# ---------------------------------------------------------------------------
+
+RANK = {"stop": 6, "D": 5, "Ve": 4, "Vo": 3, "He": 2, "Ho": 1}
+
+
+def _annotated(i, j, m, n, prev, prefix, out):
+    out.append((i, j, prefix))
+    if i < m and j < n:
+        _annotated(i + 1, j + 1, m, n, "D", prefix + ("D",), out)
+    if i < m:
+        _annotated(i + 1, j, m, n, "V", prefix + ("Vo",), out)
+        if prev == "V":
+            _annotated(i + 1, j, m, n, "V", prefix + ("Ve",), out)
+    if j < n:
+        _annotated(i, j + 1, m, n, "H", prefix + ("Ho",), out)
+        if prev == "H":
+            _annotated(i, j + 1, m, n, "H", prefix + ("He",), out)
+
+
+def annotated_paths_from(i0, j0, m, n):
+    out = []
+    _annotated(i0, j0, m, n, None, (), out)
+    return out
+
+
+def annotated_score(h, v, i0, j0, moves, mat, go, ge):
+    s, i, j = 0, i0, j0
+    for mv in moves:
+        if mv == "D":
+            s += int(mat[int(v[i]) * 32 + int(h[j])])
+            i += 1
+            j += 1
+        else:
+            s += go if mv[1] == "o" else ge
+            if mv[0] == "V":
+                i += 1
+            else:
+                j += 1
+    return s
+
+
+def _backward_key(moves):
+    # read from the end; the path's beginning reads as "stop"
+    return tuple(RANK[m] for m in reversed(moves)) + (RANK["stop"],)
+
+
+def policy_path_local(h, v, mat, go, ge, end):
+    """(i0, j0, moves) of the policy's optimal local path ending at `end`
+    (the first-maximum end cell); moves are plain D/V/H."""
+    m, n = len(v), len(h)
+    cands, top = [], None
+    for i0 in range(m + 1):
+        for j0 in range(n + 1):
+            for (i1, j1, mv) in annotated_paths_from(i0, j0, m, n):
+                if not mv or (i1, j1) != end:
+                    continue
+                s = annotated_score(h, v, i0, j0, mv, mat, go, ge)
+                if top is None or s > top:
+                    top, cands = s, []
+                if s == top:
+                    cands.append((i0, j0, mv))
+    best = max(cands, key=lambda c: _backward_key(c[2]))
+    return best[0], best[1], "".join(x[0] for x in best[2]), top
+
+
+def policy_path_global(h, v, mat, go, ge):
+    m, n = len(v), len(h)
+    cands, top = [], None
+    for (i1, j1, mv) in annotated_paths_from(0, 0, m, n):
+        if (i1, j1) != (m, n):
+            continue
+        s = annotated_score(h, v, 0, 0, mv, mat, go, ge)
+        if top is None or s > top:
+            top, cands = s, []
+        if s == top:
+            cands.append(mv)
+    best = max(cands, key=_backward_key)
+    return "".join(x[0] for x in best), top

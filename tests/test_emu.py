@@ -1,0 +1,100 @@
+"""The library's kernels and host code on the CPU, under AddressSanitizer and
+UndefinedBehaviorSanitizer, against the oracle.
+
+tests/emu compiles the HIP sources of libpastis_hip for the host, with every
+wavefront emulated by 64 threads that meet at each cross-lane operation
+(DPP, ballot, shuffle, readlane).  A cross-lane operation that only some
+lanes of a wave reach (the pattern behind a GPU hang: a lane-0-only branch
+ahead of a wave-wide step) leaves the other lanes waiting, so such a
+regression shows up here as a time-out instead of on the GPU.  The driver
+(tests/emu/driver.cpp) runs one batch through the C ABI: --sfa with
+traceback statistics, and --sxa with both X-drop kernels (one lane per item,
+and 16 lanes per item).  Results must equal the oracle's bit for bit.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from helpers import oracle_params, small_config1, small_config2
+from oracle import oracle as orc
+from pastis_amd._lib import EXTENT_DTYPE, PAIR_DTYPE, RESULT_DTYPE
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+EMU = os.path.join(HERE, "emu")
+OUT = os.path.join(EMU, "build_san")
+DRIVER = os.path.join(OUT, "driver")
+SAN = "-fsanitize=address,undefined -fno-sanitize-recover=undefined"
+
+
+@pytest.fixture(scope="module")
+def driver():
+    subprocess.run(["make", "-s", "-j8", "-C", EMU, "OUT=" + OUT,
+                    "SAN=" + SAN, DRIVER], check=True, timeout=900)
+    return DRIVER
+
+
+def run(driver, tmp_path, w, pairs, mode, xdrop=0, coop=0,
+        gap_model=0):
+    inp = tmp_path / "in.bin"
+    outp = tmp_path / "out.bin"
+    hdr = np.array([mode, -11, -2, 6, xdrop, 2, gap_model, coop], np.int32)
+    offs = np.ascontiguousarray(w.offsets, np.uint64)
+    with open(inp, "wb") as f:
+        f.write(hdr.tobytes())
+        f.write(np.array([len(offs) - 1, len(pairs)], np.uint64).tobytes())
+        f.write(offs.tobytes())
+        f.write(np.ascontiguousarray(w.codes, np.uint8).tobytes())
+        f.write(np.ascontiguousarray(pairs, PAIR_DTYPE).tobytes())
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1",
+               UBSAN_OPTIONS="print_stacktrace=1")
+    r = subprocess.run([driver, str(inp), str(outp)], env=env,
+                       capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-4000:]
+    raw = open(outp, "rb").read()
+    n = len(pairs)
+    res = np.frombuffer(raw[:n * RESULT_DTYPE.itemsize], RESULT_DTYPE)
+    ext = cells = None
+    if mode == 1:
+        o = n * RESULT_DTYPE.itemsize
+        ext = [np.frombuffer(raw[o + s * n * 8:o + (s + 1) * n * 8],
+                             EXTENT_DTYPE) for s in range(2)]
+        cells = int(np.frombuffer(raw[o + 2 * n * 8:o + 2 * n * 8 + 8],
+                                  np.uint64)[0])
+    return res, ext, cells
+
+
+def test_emulated_sfa_under_sanitizers(driver, tmp_path):
+    w = small_config1(n_seq=20, n_pairs=20, seed=71)
+    got, _, _ = run(driver, tmp_path, w, w.pairs, 0)
+    ref = orc.sfa_batch(w.codes, w.offsets, w.pairs["r"], w.pairs["c"],
+                        oracle_params("as_called"))
+    for gf, rf in [("score", "score"), ("r_end", "v_end"), ("c_end", "h_end"),
+                   ("r_beg", "v_beg"), ("c_beg", "h_beg"),
+                   ("matches", "matches"), ("cols", "cols"),
+                   ("gap_opens", "gap_opens")]:
+        assert np.array_equal(got[gf].astype(np.int64),
+                              ref[rf].astype(np.int64)), gf
+    assert np.array_equal(got["score_aln"], ref["score_aln"])
+
+
+@pytest.mark.parametrize("coop", [0, 1])
+def test_emulated_sxa_under_sanitizers(driver, tmp_path, coop):
+    w = small_config2(n_seq=20, n_pairs=16, seed=72)
+    got, ext, cells = run(driver, tmp_path, w, w.pairs, 1, xdrop=49,
+                          coop=coop)
+    ref = orc.sxa_batch(w.codes, w.offsets, w.pairs["r"], w.pairs["c"],
+                        w.pairs["seed"], oracle_params("as_called"))
+    for gf in ("score", "matches", "cols", "gap_opens"):
+        assert np.array_equal(got[gf].astype(np.int64),
+                              ref[gf].astype(np.int64)), gf
+    assert np.array_equal(got["score_aln"], ref["score_aln"])
+    for s in range(2):
+        e = ref["ext"][:, s]
+        assert np.array_equal(ext[s]["r_beg"], e["beg_v"])
+        assert np.array_equal(ext[s]["r_end"], e["end_v"])
+        assert np.array_equal(ext[s]["c_beg"], e["beg_h"])
+        assert np.array_equal(ext[s]["c_end"], e["end_h"])
+    # evaluated X-drop cells of the pa_align call
+    assert cells == int(ref["ext"]["cells"].sum())

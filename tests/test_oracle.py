@@ -83,6 +83,77 @@ def test_global_brute_force(go, ge):
                 "gap_opens": o["gap_opens"]} in stats
 
 
+@pytest.mark.parametrize("go,ge", GAPS)
+def test_local_traceback_policy(go, ge):
+    """The oracle's statistics and begin cell are those of the path the
+    documented tie policy selects among all optimal paths (bruteforce's
+    whole-path rule), not merely of some optimal path."""
+    rng = random.Random(3000 + go * 11 + ge)
+    done = 0
+    for trial in range(60):
+        alpha = ["AW", "ARNDW", STD20][trial % 3]
+        h = rnd_seq(rng, rng.randint(1, 5), alpha)
+        v = rnd_seq(rng, rng.randint(1, 5), alpha)
+        o = orc.sw_local(h, v, go, ge, stats=True)
+        if o["score"] == 0:
+            continue
+        i0, j0, mv, top = bf.policy_path_local(
+            h, v, MAT, go, ge, (o["v_end"], o["h_end"]))
+        assert top == o["score"]
+        st = bf.path_stats(h, v, i0, j0, mv)
+        assert (o["v_beg"], o["h_beg"]) == (i0, j0), (h, v, mv)
+        assert (o["matches"], o["cols"], o["gap_opens"]) == \
+            (st["matches"], st["cols"], st["gap_opens"]), (h, v, mv)
+        done += 1
+    assert done >= 30
+
+
+@pytest.mark.parametrize("go,ge", GAPS)
+def test_global_traceback_policy(go, ge):
+    rng = random.Random(4000 - go * 13 + ge)
+    for trial in range(50):
+        alpha = ["AW", "ARNDW", STD20][trial % 3]
+        h = rnd_seq(rng, rng.randint(1, 5), alpha)
+        v = rnd_seq(rng, rng.randint(1, 5), alpha)
+        o = orc.nw_global(h, v, go, ge, stats=True)
+        mv, top = bf.policy_path_global(h, v, MAT, go, ge)
+        assert top == o["score"]
+        st = bf.path_stats(h, v, 0, 0, mv)
+        assert (o["matches"], o["cols"], o["gap_opens"]) == \
+            (st["matches"], st["cols"], st["gap_opens"]), (h, v, mv)
+
+
+# Global inputs on which the policy's statistics differ from those of the
+# same rule with vertical and horizontal swapped ("hv") or with opening
+# preferred to extension ("oe"); found by searching small random inputs.
+DISCRIMINATING = [
+    ("hv", "ASA", "SAWS", -2, -11), ("hv", "SSASW", "WWWSA", -2, -11),
+    ("hv", "WAAA", "AAAWA", -3, -1), ("hv", "AAAW", "WAA", -3, -1),
+    ("hv", "WAA", "AAAW", -3, -1),
+    ("oe", "RA", "RRAAN", -5, -5), ("oe", "NRR", "RRRNR", -5, -5),
+    ("oe", "NR", "NNNA", -5, -5), ("oe", "AARRA", "NAN", -5, -5),
+    ("oe", "RNNA", "RN", -5, -5), ("oe", "NANAA", "NA", -5, -5),
+]
+SWAPPED = {"hv": {"stop": 6, "D": 5, "He": 4, "Ho": 3, "Ve": 2, "Vo": 1},
+           "oe": {"stop": 6, "D": 5, "Vo": 4, "Ve": 3, "Ho": 2, "He": 1}}
+
+
+@pytest.mark.parametrize("kind,hs,vs,go,ge", DISCRIMINATING)
+def test_traceback_policy_on_discriminating_inputs(kind, hs, vs, go, ge,
+                                                   monkeypatch):
+    """On these inputs several optimal paths have different statistics; the
+    oracle's are the documented policy's, and a swapped priority would give
+    different ones (so the policy checks above can fail)."""
+    h, v = orc.encode(hs), orc.encode(vs)
+    o = orc.nw_global(h, v, go, ge, stats=True)
+    mv, _ = bf.policy_path_global(h, v, MAT, go, ge)
+    want = tuple(bf.path_stats(h, v, 0, 0, mv).values())
+    assert (o["matches"], o["cols"], o["gap_opens"]) == want
+    monkeypatch.setattr(bf, "RANK", SWAPPED[kind])
+    mv2, _ = bf.policy_path_global(h, v, MAT, go, ge)
+    assert tuple(bf.path_stats(h, v, 0, 0, mv2).values()) != want
+
+
 def test_local_closed_forms():
     rng = random.Random(7)
     for _ in range(20):
