@@ -214,6 +214,55 @@ int pa_xdrop_extents(struct pa_ctx *ctx, const pa_pair *pairs, uint64_t n,
 int pa_kmer_prune(const pa_pair *pairs, uint64_t n, int ckthr, float mosthr,
                   uint8_t *keep, uint64_t *kept);
 
+/* ---- k-mer candidate generation ---------------------------------------
+ * Replaces the overlap matrix of the reference's similarity search: the
+ * sequence-by-k-mer matrix A of generate_A (src/kmer/KmerOps.cpp:92-245)
+ * multiplied by its transpose under the KmerIntersectLoc semiring
+ * (inc/sr.hpp:30-117; src/sim_search.cpp:154-157, 533-534), computed for the
+ * loaded row x column sets by a sort-based join on the GPU.  Each pair of
+ * sequences sharing a k-mer gets count = shared k-mer occurrence pairs
+ * (ushort, wraps), kscore = the largest BLOSUM62 self score of a shared
+ * k-mer, seed[0..1] = (row, column) offsets of the first product and
+ * seed[2..3] those of the last one ((0, 0) when count is 1), the products
+ * being added in k-mer code order, then column offset, then row offset.
+ * The symmetric pruning of the similarity search and the k-mer threshold
+ * prune of src/DistPWRunner.cpp:48-54 are fused in. */
+enum pa_alphabet {       /* --alph, inc/Alphabet.hpp:66-254 */
+    PA_ALPH_PDEFAULT = 0, PA_ALPH_MURPHY10 = 1, PA_ALPH_DSSP10 = 2,
+    PA_ALPH_GBMR10 = 3, PA_ALPH_TD10 = 4, PA_ALPH_DIAMOND = 5
+};
+enum pa_symmetric {
+    PA_SYM_NONE = 0,     /* every pair, the diagonal included */
+    PA_SYM_UPPER = 1,    /* global column > global row: mult_aln and the
+                            partial blocks of mult_aln_bl_trg,
+                            src/sim_search.cpp:403-412, 569-578 */
+    PA_SYM_EVEN_ODD = 2  /* mult_aln_bl_idx, src/sim_search.cpp:173-187:
+                            r != c, same parity below the diagonal,
+                            different parity above it */
+};
+
+typedef struct pa_kmer_config {
+    int32_t k;           /* k-mer length (-k); alphabet size ^ k < 2^31 */
+    int32_t stride;      /* k-mer stride (--ks), >= 1 */
+    int32_t alphabet;    /* enum pa_alphabet */
+    int32_t symmetric;   /* enum pa_symmetric, on global indices */
+    int32_t ckthr;       /* drop count <= ckthr (--ckthr) */
+    float mosthr;        /* ... or kscore <= mosthr (--mosthr) */
+    uint64_t row_base;   /* global index of loaded row sequence 0 */
+    uint64_t col_base;   /* global index of loaded column sequence 0 */
+} pa_kmer_config;
+
+/* Candidates of all row sequences x column sequences [c_begin, c_end) as
+ * pa_pair (r, c = loaded-set indices) in column-major order (c, then r).
+ * *n_out receives their number.  With d_out NULL only the count is
+ * computed; a second call with the same arguments (and no pa_load_seqs in
+ * between) then copies the kept result into d_out (device memory, cap
+ * entries) without recomputing; PA_ERR_LIMIT if it holds more than cap.
+ * Synchronous. */
+int pa_kmer_candidates(struct pa_ctx *ctx, const pa_kmer_config *kc,
+                       uint64_t c_begin, uint64_t c_end, pa_pair *d_out,
+                       uint64_t cap, uint64_t *n_out);
+
 /* ---- learned prefilter ------------------------------------------------
  * Stands beside the k-mer prefilter of src/DistPWRunner.cpp:48-54 (which
  * prunes on count <= ckthr || score <= mosthr): an MLP 16 -> 64 -> 64 -> 1

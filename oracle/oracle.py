@@ -244,3 +244,93 @@ def sxa_batch(seqs, offs, pr, pc, seeds, p, nthreads=0, mat=None):
     if rc:
         raise ValueError("oracle batch error")
     return out
+
+
+# ---- k-mer candidates and FASTA normalisation ----------------------------
+
+CAND_DTYPE = np.dtype([("r", "<u4"), ("c", "<u4"), ("count", "<u2"),
+                       ("kscore", "<i2"), ("seed", "<u2", (4,))])
+ALPHABETS = {"pdefault": 0, "murphy10": 1, "dssp10": 2, "gbmr10": 3,
+             "td10": 4, "diamond": 5}
+SYMMETRIC = {"none": 0, "upper": 1, "even_odd": 2}
+
+
+def _kmer_lib():
+    L = lib()
+    if not hasattr(L, "_kmer_ready"):
+        vp = C.c_void_p
+        L.or_kmer_candidates.argtypes = [vp, vp, C.c_uint64, vp, vp,
+                                         C.c_uint64, C.c_int, C.c_int,
+                                         C.c_int, C.c_int, C.c_uint64,
+                                         C.c_uint64, C.POINTER(vp),
+                                         C.POINTER(C.c_uint64)]
+        L.or_free.argtypes = [vp]
+        L.or_fasta_normalize.argtypes = [vp, C.c_uint64, C.c_int,
+                                         C.POINTER(vp), C.POINTER(vp),
+                                         C.POINTER(vp), C.POINTER(C.c_uint64)]
+        L.or_alph_tables.argtypes = [C.c_int, vp, vp, C.POINTER(C.c_int)]
+        L._kmer_ready = True
+    return L
+
+
+def alph_tables(alph="pdefault"):
+    """(code_of[128] with 255 = not a k-mer letter, self_of[128], size)."""
+    L = _kmer_lib()
+    code = np.zeros(128, np.uint8)
+    self_ = np.zeros(128, np.int16)
+    size = C.c_int()
+    assert L.or_alph_tables(ALPHABETS[alph], _ptr(code), _ptr(self_),
+                            C.byref(size)) == 0
+    return code, self_, size.value
+
+
+def _take(vp, dtype, n):
+    if n == 0:
+        lib().or_free(vp)
+        return np.zeros(0, dtype)
+    buf = (C.c_char * (n * np.dtype(dtype).itemsize)).from_address(vp.value)
+    out = np.frombuffer(bytes(buf), dtype).copy()
+    lib().or_free(vp)
+    return out
+
+
+def kmer_candidates(rres, roffs, cres=None, coffs=None, k=6, stride=1,
+                    alph="pdefault", sym="none", roff=0, coff=0):
+    """Overlap-matrix nonzeros (CAND_DTYPE, column-major) of ASCII row and
+    column sets; cres None means the column set is the row set."""
+    L = _kmer_lib()
+    rres = np.ascontiguousarray(rres, np.uint8)
+    roffs = np.ascontiguousarray(roffs, np.uint64)
+    if cres is not None:
+        cres = np.ascontiguousarray(cres, np.uint8)
+        coffs = np.ascontiguousarray(coffs, np.uint64)
+    out = C.c_void_p()
+    n = C.c_uint64()
+    rc = L.or_kmer_candidates(
+        _ptr(rres) if rres.size else None, _ptr(roffs), len(roffs) - 1,
+        None if cres is None else (_ptr(cres) if cres.size else None),
+        None if cres is None else _ptr(coffs),
+        0 if cres is None else len(coffs) - 1, k, stride, ALPHABETS[alph],
+        SYMMETRIC[sym], roff, coff, C.byref(out), C.byref(n))
+    assert rc == 0
+    return _take(out, CAND_DTYPE, n.value)
+
+
+def fasta_normalize(data, k=6):
+    """(residues uint8, offsets uint64, record index) of a FASTA file's
+    bytes as one process of the reference sees them."""
+    L = _kmer_lib()
+    data = np.frombuffer(bytes(data), np.uint8)
+    res, offs, orig = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    ns = C.c_uint64()
+    assert L.or_fasta_normalize(_ptr(data) if data.size else None, data.size,
+                                k, C.byref(res), C.byref(offs),
+                                C.byref(orig), C.byref(ns)) == 0
+    n = ns.value
+    if not offs.value:
+        return (np.zeros(0, np.uint8), np.zeros(1, np.uint64),
+                np.zeros(0, np.uint64))
+    o = _take(offs, np.uint64, n + 1)
+    r = _take(res, np.uint8, int(o[-1]))
+    g = _take(orig, np.uint64, n)
+    return r, o, g

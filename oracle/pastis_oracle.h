@@ -111,6 +111,45 @@ int or_sxa_batch(const uint8_t *seqs, const uint64_t *offs, const uint32_t *pr,
                  const int8_t *mat, const or_params_t *p, or_pair_out_t *out,
                  int nthreads);
 
+/* ---- k-mer candidates (kmer_oracle.c) ---------------------------------- */
+
+/* One nonzero of the overlap matrix, laid out like the library's pa_pair:
+ * count and score of CommonKmerLoc (ushort count wraps), seed[0..1] =
+ * first.(row offset, column offset), seed[2..3] = second. */
+typedef struct {
+    uint32_t r, c;
+    uint16_t count;
+    int16_t score;
+    uint16_t seed[4];
+} or_cand_t;
+
+/* Alphabet id 0 pdefault, 1 murphy10, 2 dssp10, 3 gbmr10, 4 td10, 5 diamond:
+ * code_of[upper-case letter] (255 = not a k-mer letter), self score of the
+ * mapped letter, and the alphabet size (the k-mer code's base). */
+int or_alph_tables(int alph, uint8_t code_of[128], int16_t self_of[128],
+                   int *size);
+
+/* Candidates of row set x column set (cres NULL: the column set is the row
+ * set), ASCII residues, k-mer length k, stride, alphabet, symmetric pruning
+ * sym (0 none, 1 strict upper triangle, 2 even/odd) on global indices
+ * r + roff, c + coff.  *out (free with or_free) holds *nout candidates in
+ * column-major order (c, then r), before the k-mer threshold prune. */
+int or_kmer_candidates(const char *rres, const uint64_t *roffs, uint64_t nr,
+                       const char *cres, const uint64_t *coffs, uint64_t nc,
+                       int k, int stride, int alph, int sym, uint64_t roff,
+                       uint64_t coff, or_cand_t **out, uint64_t *nout);
+void or_free(void *p);
+
+/* ---- FASTA normalisation (fasta_oracle.c) ------------------------------ */
+
+/* The sequences one process of the reference hands to the aligner for a
+ * FASTA file of `size` bytes: *res_out / *offs_out (nseq + 1 entries) hold
+ * the residues, *orig_out the record index of each sequence in the file.
+ * Free the three arrays with or_free. */
+int or_fasta_normalize(const char *file, uint64_t size, int k, char **res_out,
+                       uint64_t **offs_out, uint64_t **orig_out,
+                       uint64_t *nseq_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -30,8 +30,12 @@ EXPORTS = [
     "pa_filter_set_model", "pa_filter_get_model", "pa_filter_grad",
     "pa_filter_train", "pa_filter_logits", "pa_filter_score_dev",
     "pa_filter_select_dev", "pa_set_thresholds", "pa_get_tuning",
-    "pa_set_tuning", "pa_sw_scores_graph",
+    "pa_set_tuning", "pa_sw_scores_graph", "pa_kmer_candidates",
 ]
+
+ALPHABETS = {"pdefault": 0, "murphy10": 1, "dssp10": 2, "gbmr10": 3,
+             "td10": 4, "diamond": 5}
+SYMMETRIC = {"none": 0, "upper": 1, "even_odd": 2}
 
 PA_FILTER_NF = 16
 PA_FILTER_NPARAM = 5313
@@ -81,6 +85,13 @@ class PaTuning(C.Structure):
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class PaKmerConfig(C.Structure):
+    _fields_ = [("k", C.c_int32), ("stride", C.c_int32),
+                ("alphabet", C.c_int32), ("symmetric", C.c_int32),
+                ("ckthr", C.c_int32), ("mosthr", C.c_float),
+                ("row_base", C.c_uint64), ("col_base", C.c_uint64)]
 
 
 class PaTrainConfig(C.Structure):
@@ -151,6 +162,8 @@ def lib():
     L.pa_filter_score_dev.argtypes = [vp, vp, u64, vp, vp]
     L.pa_filter_select_dev.argtypes = [vp, vp, u64, C.c_float, vp, vp,
                                        C.POINTER(C.c_uint64), vp]
+    L.pa_kmer_candidates.argtypes = [vp, C.POINTER(PaKmerConfig), u64, u64,
+                                     vp, u64, C.POINTER(C.c_uint64)]
     _lib = L
     return L
 

@@ -155,6 +155,49 @@ class Aligner:
         check(self._ctx, lib().pa_set_tuning(self._ctx, C.byref(t)),
               "pa_set_tuning")
 
+    # ---- k-mer candidate generation ---------------------------------------
+    def kmer_candidates(self, k=6, stride=1, alphabet="pdefault",
+                        symmetric="none", ckthr=0, mosthr=-1.0, row_base=0,
+                        col_base=0, c_begin=0, c_end=None):
+        """Overlap-matrix nonzeros of the loaded sets (pa_kmer_candidates),
+        as PAIR_DTYPE in column-major order."""
+        kc = _lib.PaKmerConfig(k, stride, _lib.ALPHABETS[alphabet],
+                               _lib.SYMMETRIC[symmetric], ckthr, mosthr,
+                               row_base, col_base)
+        ce = (1 << 63) if c_end is None else c_end
+        n = C.c_uint64()
+        check(self._ctx, lib().pa_kmer_candidates(
+            self._ctx, C.byref(kc), c_begin, ce, None, 0, C.byref(n)),
+              "pa_kmer_candidates")
+        out = np.zeros(n.value, PAIR_DTYPE)
+        if n.value == 0:
+            return out
+        d = self.device_alloc(out.nbytes)
+        try:
+            check(self._ctx, lib().pa_kmer_candidates(
+                self._ctx, C.byref(kc), c_begin, ce, d, n.value, C.byref(n)),
+                  "pa_kmer_candidates")
+            self.d2h(out, d)
+        finally:
+            self.device_free(d)
+        return out
+
+    def kmer_candidates_dev(self, d_out, cap, k=6, stride=1,
+                            alphabet="pdefault", symmetric="none", ckthr=0,
+                            mosthr=-1.0, row_base=0, col_base=0, c_begin=0,
+                            c_end=None):
+        """Device variant: fills d_out (cap pairs); returns the count
+        (raises when it exceeds cap)."""
+        kc = _lib.PaKmerConfig(k, stride, _lib.ALPHABETS[alphabet],
+                               _lib.SYMMETRIC[symmetric], ckthr, mosthr,
+                               row_base, col_base)
+        ce = (1 << 63) if c_end is None else c_end
+        n = C.c_uint64()
+        check(self._ctx, lib().pa_kmer_candidates(
+            self._ctx, C.byref(kc), c_begin, ce, d_out, cap, C.byref(n)),
+              "pa_kmer_candidates")
+        return n.value
+
     # ---- learned prefilter ------------------------------------------------
     def filter_features(self, pairs):
         pairs = np.ascontiguousarray(pairs, PAIR_DTYPE)

@@ -601,15 +601,30 @@ int upload_set(pa_ctx *c, bool rows, const uint8_t *res, const uint64_t *offs,
         mx = std::max(mx, (uint32_t)L);
     }
     std::vector<uint8_t> codes(total + 64, (uint8_t)kSentinel);
+    // k-mer letters: a residue upper-cased with & 0xDF must be one of the 25
+    // letters (add_kmers, src/kmer/KmerOps.cpp:61-73); '*' never is, since
+    // '*' & 0xDF is not a letter
+    std::vector<uint8_t> kc(total + 64, 255);
     for (uint64_t i = 0; i < total; ++i) {
         uint8_t x = res[base + i];
-        if (enc == PA_ENC_ASCII)
+        uint8_t k = 255;
+        if (enc == PA_ENC_ASCII) {
+            const uint8_t up = (uint8_t)(x & 0xDF);
+            k = up < 128 ? encode_ascii(up) : 22;
+            if (k == 22 && up != 'X')
+                k = 255;
+            if (k == 23)
+                k = 255;
             x = encode_ascii(x);
-        else if (x >= 25)
+        } else if (x >= 25) {
             return fail(c, PA_ERR_RESIDUE,
                         "residue code " + std::to_string(x) + " at offset " +
                             std::to_string(base + i));
+        } else {
+            k = x == 23 ? 255 : x;
+        }
         codes[i] = x;
+        kc[i] = k;
     }
     // Residue-bound prefix sums (ResBound) after the codes: the best score of
     // each code against any code, in either operand position, at least 0.
@@ -630,8 +645,12 @@ int upload_set(pa_ctx *c, bool rows, const uint8_t *res, const uint64_t *offs,
     pre[total] = acc;
     Buf &cb = rows ? c->rcodes : c->ccodes;
     Buf &ob = rows ? c->roffs : c->coffs;
-    if ((rc = grow(c, cb, pre_off + pre.size() * sizeof(uint32_t))))
+    Buf &kb = rows ? c->rkc : c->ckc;
+    if ((rc = grow(c, cb, pre_off + pre.size() * sizeof(uint32_t))) ||
+        (rc = grow(c, kb, kc.size())))
         return rc;
+    HIPCHK(c, hipMemcpyAsync(kb.p, kc.data(), kc.size(), hipMemcpyHostToDevice,
+                             c->st));
     (rows ? c->rpre_off : c->cpre_off) = pre_off;
     if ((rc = grow(c, ob, (nseq + 1) * sizeof(uint64_t))))
         return rc;
@@ -830,14 +849,18 @@ void pa_destroy(pa_ctx *c)
         c->ccodes = Buf();
         c->coffs = Buf();
         c->csk = Buf();
+        c->ckc = Buf();
     }
     release(c->csk);
     filter_free(c->flt);
     c->flt = nullptr;
+    kmer_free(c->kmer);
+    c->kmer = nullptr;
     for (Buf *b : {&c->rcodes, &c->ccodes, &c->roffs, &c->coffs, &c->pairs,
                    &c->results, &c->scores, &c->tasks, &c->dpout, &c->tflags,
                    &c->sizes, &c->toffs, &c->lists, &c->sat, &c->ctr,
                    &c->trace, &c->bnd, &c->xdt, &c->xdo, &c->ovf, &c->rsk,
+                   &c->rkc, &c->ckc,
                    &c->cub, &c->cells, &c->dmat, &c->ext, &c->bincells})
         release(*b);
     for (auto e : c->free_events)
@@ -891,6 +914,7 @@ int pa_load_seqs(pa_ctx *c, int side, const uint8_t *residues,
     hipSetDevice(c->device);
     // captured graphs hold the set sizes and the boundary-row stride
     ++c->gen;
+    ++c->seq_gen;
     int rc;
     if (side == PA_SIDE_ROW || side == PA_SIDE_BOTH) {
         if (c->shared_cols) {
@@ -898,6 +922,7 @@ int pa_load_seqs(pa_ctx *c, int side, const uint8_t *residues,
             c->ccodes = Buf();
             c->coffs = Buf();
             c->csk = Buf();
+            c->ckc = Buf();
             c->have_cols = false;
             c->shared_cols = false;
         }
@@ -909,6 +934,7 @@ int pa_load_seqs(pa_ctx *c, int side, const uint8_t *residues,
             c->ccodes = Buf();
             c->coffs = Buf();
             c->csk = Buf();
+            c->ckc = Buf();
             c->shared_cols = false;
         }
         if ((rc = upload_set(c, false, residues, offsets, nseq, encoding)))
@@ -919,10 +945,12 @@ int pa_load_seqs(pa_ctx *c, int side, const uint8_t *residues,
             release(c->ccodes);
             release(c->coffs);
             release(c->csk);
+            release(c->ckc);
         }
         c->ccodes = c->rcodes;
         c->coffs = c->roffs;
         c->csk = c->rsk;
+        c->ckc = c->rkc;
         c->cpre_off = c->rpre_off;
         c->clen = c->rlen;
         c->max_clen = c->max_rlen;

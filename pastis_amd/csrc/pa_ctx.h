@@ -21,6 +21,8 @@ struct Timer {
 
 struct FilterState;   // learned prefilter (pa_filter.hip)
 void filter_free(FilterState *f);
+struct KmerState;     // k-mer candidate generation (k_kmer.hip)
+void kmer_free(KmerState *k);
 
 }  // namespace pa
 
@@ -41,6 +43,11 @@ struct pa_ctx {
     std::string err;
     // sequence sets
     Buf rcodes, ccodes, roffs, coffs;
+    // k-mer letters of the sets, one byte per residue at the residue's
+    // offset in rcodes / ccodes: the letter's code 0..24, or 255 for a
+    // letter that no k-mer may hold (KmerOps.cpp:66-73)
+    Buf rkc, ckc;
+    uint64_t seq_gen = 0;         // bumped whenever a sequence set changes
     Buf rsk, csk;                 // per-sequence filter sketches
     // byte offset of the residue-bound prefix sums inside rcodes / ccodes
     size_t rpre_off = 0, cpre_off = 0;
@@ -62,6 +69,7 @@ struct pa_ctx {
     std::vector<hipEvent_t> free_events;
     pa_stats stats{};
     pa::FilterState *flt = nullptr;
+    pa::KmerState *kmer = nullptr;
     int ncu = 256;                // compute units of the device
     // hipGraph of one score-only SW batch (pa_sw_scores_graph)
     bool capturing = false;       // no event timers while a stream captures

@@ -9,6 +9,7 @@
 //         uint8 codes[offsets[nseq]]; pa_pair pairs[npairs]
 // output: pa_result results[npairs], then (X-drop mode) pa_extent of seed 0
 //         and of seed 1 for every pair, then the uint64 X-drop cell count.
+// Mode 2 runs the k-mer candidate generation instead (see kmer_mode).
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -27,6 +28,51 @@ int fail(const char *what, int rc, pa_ctx *c)
     std::fprintf(stderr, "driver: %s: %s (%s)\n", what, pa_strerror(rc),
                  c ? pa_last_error(c) : "");
     return 1;
+}
+
+// mode 2: k-mer candidates of the set against itself.  h[1] k, h[2] stride,
+// h[3] alphabet, h[4] symmetric rule, h[5] ckthr, h[6] encoding; output:
+// uint64 count, then the pa_pair candidates.
+int kmer_mode(const int32_t *h, const std::vector<uint64_t> &offs,
+              const std::vector<uint8_t> &res, const char *outp)
+{
+    pa_config cfg;
+    pa_config_for_cli(&cfg, -11, -2, 6, 0, 2, PA_GAP_AS_CALLED);
+    pa_ctx *c = nullptr;
+    int rc;
+    if ((rc = pa_create(&cfg, &c)))
+        return fail("pa_create", rc, nullptr);
+    const uint64_t nseq = offs.size() - 1;
+    if ((rc = pa_load_seqs(c, PA_SIDE_BOTH, res.data(), offs.data(), nseq,
+                           h[6])))
+        return fail("pa_load_seqs", rc, c);
+    pa_kmer_config kc{};
+    kc.k = h[1];
+    kc.stride = h[2];
+    kc.alphabet = h[3];
+    kc.symmetric = h[4];
+    kc.ckthr = h[5];
+    kc.mosthr = -1.0f;
+    uint64_t n = 0;
+    if ((rc = pa_kmer_candidates(c, &kc, 0, nseq, nullptr, 0, &n)))
+        return fail("pa_kmer_candidates", rc, c);
+    void *d = nullptr;
+    if ((rc = pa_device_alloc(c, n * sizeof(pa_pair) + 1, &d)))
+        return fail("pa_device_alloc", rc, c);
+    if ((rc = pa_kmer_candidates(c, &kc, 0, nseq, (pa_pair *)d, n, &n)))
+        return fail("pa_kmer_candidates", rc, c);
+    std::vector<pa_pair> out(n);
+    if (n && (rc = pa_memcpy_d2h(c, out.data(), d, n * sizeof(pa_pair))))
+        return fail("pa_memcpy_d2h", rc, c);
+    FILE *f = std::fopen(outp, "wb");
+    if (!f)
+        return 2;
+    std::fwrite(&n, sizeof(n), 1, f);
+    std::fwrite(out.data(), sizeof(pa_pair), n, f);
+    std::fclose(f);
+    pa_device_free(c, d);
+    pa_destroy(c);
+    return 0;
 }
 
 }  // namespace
@@ -53,6 +99,8 @@ int main(int argc, char **argv)
         return 2;
     std::fclose(in);
 
+    if (h[0] == 2)
+        return kmer_mode(h, offs, codes, argv[2]);
     pa_config cfg;
     int rc = pa_config_for_cli(&cfg, h[1], h[2], h[3], h[4], h[5], h[6]);
     if (rc)

@@ -98,3 +98,43 @@ def test_emulated_sxa_under_sanitizers(driver, tmp_path, coop):
         assert np.array_equal(ext[s]["c_end"], e["end_h"])
     # evaluated X-drop cells of the pa_align call
     assert cells == int(ref["ext"]["cells"].sum())
+
+
+def run_kmer(driver, tmp_path, strs, k=3, stride=1, alph=0, sym=0, ckthr=0):
+    inp = tmp_path / "k_in.bin"
+    outp = tmp_path / "k_out.bin"
+    res = np.frombuffer("".join(strs).encode(), np.uint8)
+    offs = np.zeros(len(strs) + 1, np.uint64)
+    offs[1:] = np.cumsum([len(s) for s in strs])
+    hdr = np.array([2, k, stride, alph, sym, ckthr, 1, 0], np.int32)
+    with open(inp, "wb") as f:
+        f.write(hdr.tobytes())
+        f.write(np.array([len(strs), 0], np.uint64).tobytes())
+        f.write(offs.tobytes())
+        f.write(res.tobytes())
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1")
+    r = subprocess.run([driver, str(inp), str(outp)], env=env,
+                       capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-4000:]
+    raw = open(outp, "rb").read()
+    n = int(np.frombuffer(raw[:8], np.uint64)[0])
+    return np.frombuffer(raw[8:8 + n * PAIR_DTYPE.itemsize], PAIR_DTYPE), \
+        res, offs
+
+
+@pytest.mark.parametrize("alph,sym,ckthr", [(0, 0, 0), (1, 1, 1), (5, 2, 0)])
+def test_emulated_kmer_candidates_under_sanitizers(driver, tmp_path, alph,
+                                                   sym, ckthr):
+    import random
+    rng = random.Random(alph * 7 + sym)
+    strs = ["".join(rng.choice("ACDEWKLu") for _ in range(rng.randint(0, 30)))
+            for _ in range(14)]
+    got, res, offs = run_kmer(driver, tmp_path, strs, k=3, alph=alph,
+                              sym=sym, ckthr=ckthr)
+    name = {v: k for k, v in orc.ALPHABETS.items()}[alph]
+    symn = {v: k for k, v in orc.SYMMETRIC.items()}[sym]
+    ref = orc.kmer_candidates(res, offs, k=3, alph=name, sym=symn)
+    ref = ref[~(ref["count"] <= ckthr)]
+    assert len(got) == len(ref)
+    for f in ("r", "c", "count", "kscore", "seed"):
+        assert np.array_equal(got[f], ref[f]), f
