@@ -204,6 +204,17 @@ int pa_sw_scores_dev(struct pa_ctx *ctx, const pa_pair *d_pairs, uint64_t n,
 int pa_sw_scores_graph(struct pa_ctx *ctx, const pa_pair *d_pairs, uint64_t n,
                        pa_score *d_out, void *stream);
 
+/* pa_sw_scores_dev without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * allocation).  The call's workspace is a fixed 64 MiB device array of the
+ * library (one per GPU, held until the process ends); it takes at most
+ * 262144 pairs per call and loaded sequences of at most 4096 residues
+ * (PA_ERR_LIMIT otherwise), runs no event timers, and calls on one GPU that
+ * overlap in time race on that array.  Results equal pa_sw_scores_dev's. */
+int pa_sw_scores_noalloc(struct pa_ctx *ctx, const pa_pair *d_pairs,
+                         uint64_t n, pa_score *d_out, void *stream);
+
 /* Seed extension only: extendSeed(..., GappedXDrop) of
  * SeqanXdropAligner.cpp:279-289 for seed `seed` (0 or 1) of every pair. */
 int pa_xdrop_extents(struct pa_ctx *ctx, const pa_pair *pairs, uint64_t n,
@@ -262,6 +273,52 @@ typedef struct pa_kmer_config {
 int pa_kmer_candidates(struct pa_ctx *ctx, const pa_kmer_config *kc,
                        uint64_t c_begin, uint64_t c_end, pa_pair *d_out,
                        uint64_t cap, uint64_t *n_out);
+
+/* ---- input and output formats -----------------------------------------
+ * FASTA normalisation: the sequences one process of the reference hands to
+ * its aligners for a FASTA file (the one-rank reader of
+ * src/DistFastaData.cpp:161-263 drops the file's last byte; the FastaData
+ * constructor, src/FastaData.cpp:13-148, removes newlines inside sequences
+ * and '*', turns U/u into T, drops sequences shorter than k -- taking the
+ * previous sequence's last residue with a dropped record that is not the
+ * last -- and get_sequence, inc/FastaData.hpp:105-129, reads lengths as
+ * ushort).  Residues stay ASCII (load them with PA_ENC_ASCII). */
+struct pa_fasta;
+int pa_fasta_parse(const char *data, uint64_t size, int k,
+                   struct pa_fasta **out);
+/* counts: sequences, residues, sequences whose length wrapped past 65535 */
+int pa_fasta_info(const struct pa_fasta *f, uint64_t *nseq, uint64_t *nres,
+                  uint64_t *nwrapped);
+const char *pa_fasta_residues(const struct pa_fasta *f);
+const uint64_t *pa_fasta_offsets(const struct pa_fasta *f);   /* nseq + 1 */
+const uint64_t *pa_fasta_records(const struct pa_fasta *f);   /* record
+                                  index in the file of each sequence */
+void pa_fasta_free(struct pa_fasta *f);
+/* pa_fasta_parse + pa_load_seqs(PA_ENC_ASCII): construct_seqs from a FASTA
+ * buffer (SeqanXdropAligner.cpp:32-105 over DistFastaData). */
+int pa_load_fasta(struct pa_ctx *ctx, int side, const char *data,
+                  uint64_t size, int k);
+
+/* One nonzero of the similarity matrix: global row, column, score_aln. */
+typedef struct pa_triple {
+    uint64_t row, col;
+    float score_aln;
+    uint32_t pad;
+} pa_triple;
+
+/* The pairs whose alignment passed (score_aln != 0, the post-prune of
+ * src/DistPWRunner.cpp:149-153) as triples (r + row_base, c + col_base,
+ * score_aln) in the pairs' order, collected like src/sim_search.cpp:223-232.
+ * d_out holds up to n triples; *n_out receives their number.  Synchronous. */
+int pa_triples_dev(struct pa_ctx *ctx, const pa_pair *d_pairs,
+                   const pa_result *d_res, uint64_t n, uint64_t row_base,
+                   uint64_t col_base, pa_triple *d_out, uint64_t *n_out,
+                   void *stream);
+/* Writes the similarity matrix as Matrix Market (coordinate real general,
+ * 1-based, column-major order, %g values), the output of
+ * S.ParallelWriteMM(file, true) (src/sim_search.cpp:257-269). */
+int pa_write_mm(const char *path, uint64_t nrows, uint64_t ncols,
+                const pa_triple *triples, uint64_t n);
 
 /* ---- learned prefilter ------------------------------------------------
  * Stands beside the k-mer prefilter of src/DistPWRunner.cpp:48-54 (which

@@ -17,9 +17,10 @@
  *     [seq_start_i, id_start_{i+1} - 1), the last one [seq_start, l_end + 1),
  *     and its length is taken as a ushort (mod 65536).
  * The restatement follows the constructor's loop over a copy of the buffer
- * with one byte of room in front: when the first record is removed the
- * reference writes one byte before its buffer (undefined behaviour); here
- * that byte lands in the room and the other records are unaffected.
+ * with room in front: when the first record is removed the reference writes
+ * one byte before its buffer (undefined behaviour), one more for every
+ * further record removed at the start; here those bytes land in the room and
+ * the other records are unaffected.
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -38,8 +39,10 @@ int or_fasta_normalize(const char *file, uint64_t size, int k, char **res_out,
     if (size < 2)
         return 0;
     const uint64_t n = size - 1;   /* bytes [0, size-2] */
-    char *room = (char *)malloc(n + 2);
-    char *buff = room + 1;
+    /* each erased record at the start reaches one byte further before the
+     * buffer; there are fewer records than bytes */
+    char *room = (char *)malloc(2 * n + 2);
+    char *buff = room + n + 1;
     memcpy(buff, file, n);
     buff[n] = 0;
     /* signed positions: a removed first record moves one byte before 0 */
@@ -85,7 +88,10 @@ int or_fasta_normalize(const char *file, uint64_t size, int k, char **res_out,
             ++seq_len;
         }
     }
-    if (seq_len < k && nid > 0 && nss > 0) {
+    /* a file that ends inside a header line leaves a record without a
+     * sequence; the reference then pops mismatched entries (undefined), so
+     * the removal is only restated for a complete last record */
+    if (seq_len < k && nid > 0 && nss == nid) {
         nc += seq_len + (seq_starts[nss - 1] - id_starts[nid - 1]) + 1;
         --nid;
         --nss;
@@ -97,7 +103,10 @@ int or_fasta_normalize(const char *file, uint64_t size, int k, char **res_out,
     offs[0] = 0;
     for (uint64_t s = 0; s < ns; ++s) {
         const int64_t end_pos = s + 1 < nid ? id_starts[s + 1] - 1 : l_end2 + 1;
-        const uint16_t len = (uint16_t)(end_pos - seq_starts[s]);
+        /* erased records eating past this sequence's start: no residues
+         * (the reference reads a wrapped length there) */
+        const int64_t span = end_pos > seq_starts[s] ? end_pos - seq_starts[s] : 0;
+        const uint16_t len = (uint16_t)span;
         tot += len;
         offs[s + 1] = tot;
     }

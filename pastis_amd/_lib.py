@@ -31,7 +31,13 @@ EXPORTS = [
     "pa_filter_train", "pa_filter_logits", "pa_filter_score_dev",
     "pa_filter_select_dev", "pa_set_thresholds", "pa_get_tuning",
     "pa_set_tuning", "pa_sw_scores_graph", "pa_kmer_candidates",
+    "pa_fasta_parse", "pa_fasta_info", "pa_fasta_residues",
+    "pa_fasta_offsets", "pa_fasta_records", "pa_fasta_free", "pa_load_fasta",
+    "pa_triples_dev", "pa_write_mm", "pa_sw_scores_noalloc",
 ]
+
+TRIPLE_DTYPE = np.dtype([("row", "<u8"), ("col", "<u8"),
+                         ("score_aln", "<f4"), ("pad", "<u4")])
 
 ALPHABETS = {"pdefault": 0, "murphy10": 1, "dssp10": 2, "gbmr10": 3,
              "td10": 4, "diamond": 5}
@@ -102,6 +108,7 @@ class PaTrainConfig(C.Structure):
 
 
 assert PAIR_DTYPE.itemsize == 20 and RESULT_DTYPE.itemsize == 36
+assert TRIPLE_DTYPE.itemsize == 24
 assert SCORE_DTYPE.itemsize == 8 and EXTENT_DTYPE.itemsize == 8
 
 _lib = None
@@ -133,7 +140,8 @@ def lib():
     L.pa_load_seqs.argtypes = [vp, i32, vp, vp, u64, i32]
     for f in ("pa_align", "pa_sw_scores"):
         getattr(L, f).argtypes = [vp, vp, u64, vp]
-    for f in ("pa_align_dev", "pa_sw_scores_dev", "pa_sw_scores_graph"):
+    for f in ("pa_align_dev", "pa_sw_scores_dev", "pa_sw_scores_graph",
+              "pa_sw_scores_noalloc"):
         getattr(L, f).argtypes = [vp, vp, u64, vp, vp]
     L.pa_xdrop_extents.argtypes = [vp, vp, u64, i32, vp]
     L.pa_kmer_prune.argtypes = [vp, u64, i32, C.c_float, vp,
@@ -164,6 +172,18 @@ def lib():
                                        C.POINTER(C.c_uint64), vp]
     L.pa_kmer_candidates.argtypes = [vp, C.POINTER(PaKmerConfig), u64, u64,
                                      vp, u64, C.POINTER(C.c_uint64)]
+    L.pa_fasta_parse.argtypes = [vp, u64, i32, C.POINTER(vp)]
+    L.pa_fasta_info.argtypes = [vp, C.POINTER(C.c_uint64),
+                                C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    for f in ("pa_fasta_residues", "pa_fasta_offsets", "pa_fasta_records"):
+        getattr(L, f).argtypes = [vp]
+        getattr(L, f).restype = vp
+    L.pa_fasta_free.argtypes = [vp]
+    L.pa_fasta_free.restype = None
+    L.pa_load_fasta.argtypes = [vp, i32, vp, u64, i32]
+    L.pa_triples_dev.argtypes = [vp, vp, vp, u64, u64, u64, vp,
+                                 C.POINTER(C.c_uint64), vp]
+    L.pa_write_mm.argtypes = [C.c_char_p, u64, u64, vp, u64]
     _lib = L
     return L
 

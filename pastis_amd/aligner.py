@@ -114,6 +114,11 @@ class Aligner:
         check(self._ctx, lib().pa_sw_scores_dev(self._ctx, d_pairs, n, d_out,
                                                 None), "pa_sw_scores_dev")
 
+    def sw_scores_noalloc(self, d_pairs, n, d_out):
+        check(self._ctx, lib().pa_sw_scores_noalloc(self._ctx, d_pairs, n,
+                                                    d_out, None),
+              "pa_sw_scores_noalloc")
+
     def sw_scores_graph(self, d_pairs, n, d_out):
         check(self._ctx, lib().pa_sw_scores_graph(self._ctx, d_pairs, n, d_out,
                                                   None), "pa_sw_scores_graph")

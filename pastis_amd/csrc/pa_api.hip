@@ -117,7 +117,7 @@ struct Scope {
     hipStream_t s;
     bool on;
     Scope(pa_ctx *c_, int kind, hipStream_t s_)
-        : c(c_), s(s_), on(!c_->capturing)
+        : c(c_), s(s_), on(!c_->capturing && !c_->quiet)
     {
         if (!on)
             return;
@@ -188,6 +188,8 @@ int dp_grid(pa_ctx *c, uint64_t ntasks, int cls, bool local, bool trace,
     int per_cu = dp_occupancy(cls, local, trace, i16);
     if (c->tune.dp_wg_per_cu > 0)
         per_cu = std::min(per_cu, (int)c->tune.dp_wg_per_cu);
+    if (c->wg_cap > 0)
+        per_cu = std::min(per_cu, c->wg_cap);
     per_cu = std::min(per_cu, kMaxWgPerCu);
     return (int)std::max<uint64_t>(
         1, std::min<uint64_t>(ntasks, (uint64_t)c->ncu * per_cu));
@@ -205,9 +207,11 @@ int xd_grid(pa_ctx *c, uint64_t nitems)
 
 int ensure_bnd(pa_ctx *c)
 {
-    const size_t need =
-        (size_t)c->ncu * kMaxWgPerCu * 2u * (size_t)(c->max_clen + 2) *
-        sizeof(int32_t);
+    // one boundary row per persistent workgroup the DP grids can hold
+    const int wg = c->wg_cap > 0 ? std::min(c->wg_cap, kMaxWgPerCu)
+                                 : kMaxWgPerCu;
+    const size_t need = (size_t)c->ncu * wg * 2u *
+                        (size_t)(c->max_clen + 2) * sizeof(int32_t);
     return grow(c, c->bnd, need);
 }
 
@@ -823,6 +827,7 @@ int pa_create(const pa_config *cfg, pa_ctx **out)
         pa_destroy(c);
         return PA_ERR_NOMEM;
     }
+    noalloc_init(c->device);
     size_t fr = 0, tot = 0;
     hipMemGetInfo(&fr, &tot);
     // the trace store may take up to 40 % of free HBM (96 GB at most)
@@ -860,7 +865,7 @@ void pa_destroy(pa_ctx *c)
                    &c->results, &c->scores, &c->tasks, &c->dpout, &c->tflags,
                    &c->sizes, &c->toffs, &c->lists, &c->sat, &c->ctr,
                    &c->trace, &c->bnd, &c->xdt, &c->xdo, &c->ovf, &c->rsk,
-                   &c->rkc, &c->ckc,
+                   &c->rkc, &c->ckc, &c->tri_flag, &c->tri_pos,
                    &c->cub, &c->cells, &c->dmat, &c->ext, &c->bincells})
         release(*b);
     for (auto e : c->free_events)
@@ -1006,18 +1011,12 @@ int pa_align(pa_ctx *c, const pa_pair *pairs, uint64_t n, pa_result *out)
     return PA_OK;
 }
 
-int pa_sw_scores_dev(pa_ctx *c, const pa_pair *d_pairs, uint64_t n,
-                     pa_score *d_out, void *stream)
+}  // extern "C"
+
+int pa::sw_scores_impl(pa_ctx *c, const pa_pair *d_pairs, uint64_t n,
+                       pa_score *d_out, hipStream_t s)
 {
     int rc;
-    if ((rc = check_ready(c)))
-        return rc;
-    if (n == 0)
-        return PA_OK;
-    if (!d_pairs || !d_out)
-        return fail(c, PA_ERR_ARG, "null pointer");
-    hipSetDevice(c->device);
-    hipStream_t s = pick(c, stream);
     if ((rc = ensure_common(c, n)))
         return rc;
     Scope tot(c, 4, s);
@@ -1030,6 +1029,22 @@ int pa_sw_scores_dev(pa_ctx *c, const pa_pair *d_pairs, uint64_t n,
     HIPCHK(c, scores_out(P<DpOut>(c->dpout), n, d_out, s));
     c->stats.pairs += n;
     return PA_OK;
+}
+
+extern "C" {
+
+int pa_sw_scores_dev(pa_ctx *c, const pa_pair *d_pairs, uint64_t n,
+                     pa_score *d_out, void *stream)
+{
+    int rc;
+    if ((rc = check_ready(c)))
+        return rc;
+    if (n == 0)
+        return PA_OK;
+    if (!d_pairs || !d_out)
+        return fail(c, PA_ERR_ARG, "null pointer");
+    hipSetDevice(c->device);
+    return sw_scores_impl(c, d_pairs, n, d_out, pick(c, stream));
 }
 
 int pa_sw_scores_graph(pa_ctx *c, const pa_pair *d_pairs, uint64_t n,

@@ -61,6 +61,7 @@ struct pa_ctx {
     size_t trace_cap = 0;
     std::vector<uint64_t> h_toffs;
     Buf bincells;                 // u64 cells per (DP mode, bin)
+    Buf tri_flag, tri_pos;        // similarity-triple compaction
     struct KStat {
         uint64_t launches = 0;
         double ms = 0;
@@ -83,6 +84,11 @@ struct pa_ctx {
     // shapes change; a graph captured at another generation is recaptured.
     uint64_t gen = 0, g_gen = 0;
     pa_tuning tune{0, 0, 16, 0};
+    // No event timers (pa_sw_scores_noalloc: event objects are allocated).
+    bool quiet = false;
+    // Cap on persistent DP workgroups per CU (0 = none); the boundary-row
+    // store is sized for it.
+    int wg_cap = 0;
 };
 
 namespace pa {
@@ -131,5 +137,13 @@ inline void release(Buf &b)
 }
 
 template <class T> inline T *P(Buf &b) { return (T *)b.p; }
+
+// Score-only local alignment of n device pairs on the context's workspace
+// (pa_sw_scores_dev after its argument checks).
+int sw_scores_impl(pa_ctx *c, const pa_pair *d_pairs, uint64_t n,
+                   pa_score *d_out, hipStream_t s);
+// Resolves this device's scratch array for pa_sw_scores_noalloc
+// (pa_noalloc.hip); called once per context at creation.
+void noalloc_init(int device);
 
 }  // namespace pa

@@ -239,6 +239,78 @@ void *sg_new_set(uint64_t nseq, int model, double a, double b, int clip_lo,
     return S;
 }
 
+// Families of random sizes: sequences are drawn log-normal (median a,
+// sigma b, clipped to [clip_lo, clip_hi]); families of U[fam_lo, fam_hi]
+// members (a root and mutated children at identity U[id_lo, id_hi]) are
+// formed until they hold fam_frac of the nseq sequences, the rest are
+// unrelated roots.  Families come first, in order.
+void *sg_new_families(uint64_t nseq, double a, double b, int clip_lo,
+                      int clip_hi, double fam_frac, int fam_lo, int fam_hi,
+                      double id_lo, double id_hi, double indel_p,
+                      uint64_t seed)
+{
+    Set *S = new Set();
+    Sampler sm(seed);
+    S->offs.push_back(0);
+    S->family.resize(nseq);
+    S->root.resize(nseq);
+    S->to_root.resize(nseq);
+    std::vector<uint8_t> child;
+    std::vector<int32_t> map;
+    double id_sum = 0;
+    uint64_t id_n = 0;
+    const uint64_t in_fam = (uint64_t)std::llround(fam_frac * (double)nseq);
+    uint64_t i = 0;
+    int fam = 0;
+    auto add_root = [&](uint64_t at) {
+        int L = draw_len(sm, 1, a, b, 0.0, 0.0, 0.0);
+        L = std::min(std::max(L, clip_lo), clip_hi);
+        for (int k = 0; k < L; ++k)
+            S->codes.push_back((uint8_t)sm.bg(sm.rng));
+        S->to_root[at].resize(L);
+        for (int k = 0; k < L; ++k)
+            S->to_root[at][k] = k;
+        S->offs.push_back(S->codes.size());
+    };
+    while (i < nseq) {
+        uint64_t size = 1;
+        if (i < in_fam) {
+            size = (uint64_t)sm.uint(fam_lo, fam_hi);
+            size = std::min<uint64_t>(size, in_fam - i);
+            if (size < 2 && i + 1 < in_fam)
+                size = 2;
+        }
+        const uint64_t r = i;
+        if ((int)S->members.size() <= fam)
+            S->members.resize(fam + 1);
+        for (uint64_t m = 0; m < size && i < nseq; ++m, ++i) {
+            S->family[i] = fam;
+            S->root[i] = (int32_t)r;
+            S->members[fam].push_back((int32_t)i);
+            if (m == 0) {
+                add_root(i);
+                continue;
+            }
+            const double ident = id_lo + sm.uni() * (id_hi - id_lo);
+            id_sum += ident;
+            ++id_n;
+            const uint8_t *rp = S->codes.data() + S->offs[r];
+            const int rn = (int)(S->offs[r + 1] - S->offs[r]);
+            mutate(sm, rp, rn, ident, indel_p, child, map);
+            if (child.size() > 65535) {
+                child.resize(65535);
+                map.resize(65535);
+            }
+            S->codes.insert(S->codes.end(), child.begin(), child.end());
+            S->to_root[i] = map;
+            S->offs.push_back(S->codes.size());
+        }
+        ++fam;
+    }
+    S->mean_identity = id_n ? id_sum / id_n : 0;
+    return S;
+}
+
 // Candidate pairs: frac_hom of them within a family (r != c), the rest
 // uniform (zipf_s <= 0) or with both ends drawn from a Zipf(zipf_s) law over a
 // random permutation of the sequences; r != c always.

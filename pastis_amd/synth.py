@@ -24,6 +24,8 @@ def gen():
         G.sg_new_set.argtypes = [u64, i, d, d, i, i, d, d, d, i, d, d, d, u64]
         G.sg_new_set.restype = C.c_void_p
         G.sg_make_pairs.argtypes = [C.c_void_p, u64, d, d, i, i, u64]
+        G.sg_new_families.argtypes = [u64, d, d, i, i, d, i, i, d, d, d, u64]
+        G.sg_new_families.restype = C.c_void_p
         for f in ("sg_num_seqs", "sg_total_residues", "sg_num_pairs"):
             getattr(G, f).argtypes = [C.c_void_p]
             getattr(G, f).restype = u64
@@ -105,3 +107,36 @@ def config3(scale=1.0, seed=103, threads=8):
                 model="lognormal", a=300, b=0.6, clip=(30, 3000),
                 tail=(0.01, 3000, 35000), fam_size=1, frac_hom=0.0, zipf=1.1,
                 seed=seed, threads=threads, name="sw-heavy-tail")
+
+
+def families(nseq, *, median=280, sigma=0.6, clip=(30, 3000), fam_frac=0.05,
+             fam_size=(2, 50), ident=(0.3, 0.9), indel_p=0.05, seed=105,
+             name="families"):
+    """Sequences only (no pairs): log-normal lengths, fam_frac of them in
+    families of U[fam_size] members at identity U[ident]."""
+    G = gen()
+    h = G.sg_new_families(nseq, float(median), float(sigma), clip[0], clip[1],
+                          float(fam_frac), fam_size[0], fam_size[1], ident[0],
+                          ident[1], indel_p, seed)
+    try:
+        ns, nr = G.sg_num_seqs(h), G.sg_total_residues(h)
+        codes = np.zeros(nr, np.uint8)
+        offs = np.zeros(ns + 1, np.uint64)
+        G.sg_copy_seqs(h, codes.ctypes.data, offs.ctypes.data)
+        fam = np.zeros(ns, np.int32)
+        G.sg_copy_family(h, fam.ctypes.data)
+        info = dict(name=name, nseq=int(ns), residues=int(nr), seed=seed,
+                    mean_child_identity=G.sg_mean_identity(h))
+    finally:
+        G.sg_free(h)
+    return Workload(name, codes, offs, np.zeros(0, PAIR_DTYPE), fam, info)
+
+
+def config5(scale=1.0, seed=105):
+    """End-to-end candidate stream: 1M proteins, log-normal lengths (median
+    280, sigma 0.6 assumed, clipped to [30, 3000]), 5 % of them in families of
+    2-50 members at 30-90 % identity.  The candidates come from the k-mer
+    generator (k = 6), not from here."""
+    return families(max(int(1_000_000 * scale), 200), median=280, sigma=0.6,
+                    fam_frac=0.05, fam_size=(2, 50), ident=(0.3, 0.9),
+                    seed=seed, name="e2e-families")

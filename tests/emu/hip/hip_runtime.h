@@ -29,6 +29,10 @@
 #define PA_GLOBAL   // host memory has one address space
 #define __forceinline__ inline
 #define HIP_DYNAMIC_SHARED(type, var) type *var = (type *)emu::dyn_shared();
+#define __align__(n) __attribute__((aligned(n)))
+// a device variable is a host variable here: its "device address" is its
+// address
+#define HIP_SYMBOL(x) (&(x))
 
 struct dim3 {
     unsigned x, y, z;
@@ -248,6 +252,11 @@ template <class F>
 hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int *n, F, int, size_t)
 {
     *n = 2;
+    return hipSuccess;
+}
+inline hipError_t hipGetSymbolAddress(void **p, const void *sym)
+{
+    *p = const_cast<void *>(sym);
     return hipSuccess;
 }
 hipError_t hipSetDevice(int);
