@@ -10,8 +10,20 @@ X = 49.  Each step runs the X-drop extension of both seeds, the global
 alignment of the extended infixes with traceback statistics, the best-seed
 choice and the coverage/identity predicate.
 
-Prints one JSON line.  --workload sw runs the plain local Smith-Waterman
-configuration (2,000 proteins U[50,500], 100,000 pairs, score and end cell).
+Prints one JSON line.  Besides the headline it carries, under "configs",
+the other configurations measured the same way (fewer steps, each with its
+dominant kernel's roofline and a CPU sample):
+  C1  plain local Smith-Waterman: 2,000 proteins U[50,500], 100,000 pairs,
+      score and end cell;
+  C3  batched uneven-length Smith-Waterman: 200,000 proteins (log-normal
+      median 300 plus a 1 % tail of 3,000-35,000 residues), 5e7 Zipf pairs
+      resident in HBM, replayed as hipGraph-captured batches of 1e7 pairs;
+  C5  the end-to-end candidate stream: 1,000,000 proteins (log-normal median
+      280, 5 % in families of 2-50), k = 6 k-mer candidates (strict upper
+      triangle, ckthr = 1) generated on the GPU, then learned filter + exact
+      X-drop + triples against exact X-drop + triples on every candidate.
+--configs none skips them; --workload sw / sw3 runs C1 / C3 as the
+headline instead.
 """
 import argparse
 import json
@@ -239,6 +251,268 @@ def learned_filter(args, cfg, al, d_pairs, n, plain_out, plain_ms):
     }
 
 
+def roofline_of(al, steps_ms):
+    st = al.stats()
+    return roofline(al.kernel_stats(), st, steps_ms, al.tuning()), st
+
+
+def measure_sw_config(args, which):
+    """C1 (uniform lengths, one batch) or C3 (heavy tail, 5e7 pairs in
+    hipGraph-captured batches of 1e7) as score-only local alignment with
+    the pairs resident in HBM.  The roofline comes from one eager pass with
+    kernel timers; the timed steps replay the graphs (C3) or run eagerly
+    (C1)."""
+    import ctypes as C
+    from pastis_amd import synth
+    from pastis_amd._lib import PAIR_DTYPE, SCORE_DTYPE
+    from pastis_amd.aligner import Aligner, config_for_cli
+    t = time.time()
+    if which == "c1":
+        w = synth.config1(scale=1.0, threads=host_cores())
+        batch = len(w.pairs)
+    else:
+        w = synth.config3(scale=args.c3_scale, threads=host_cores())
+        batch = 10_000_000
+    pairs = w.pairs
+    n = len(pairs)
+    log("%s: %d sequences, %d pairs, %.3g cells, generated in %.1f s" %
+        (w.name, len(w.offsets) - 1, n, float(w.cells()), time.time() - t))
+    al = Aligner(config_for_cli(-11, -2, 6, 0, 2, args.gap_model))
+    al.load_seqs(w.codes, w.offsets)
+    d_pairs = al.device_alloc(pairs.nbytes)
+    d_out = al.device_alloc(n * SCORE_DTYPE.itemsize)
+    al.h2d(d_pairs, pairs)
+    offs = list(range(0, n, batch))
+
+    def run(graph):
+        for off in offs:
+            m = min(batch, n - off)
+            dp = C.c_void_p(d_pairs.value + off * PAIR_DTYPE.itemsize)
+            do = C.c_void_p(d_out.value + off * SCORE_DTYPE.itemsize)
+            if graph:
+                al.sw_scores_graph(dp, m, do)
+            else:
+                al.sw_scores_dev(dp, m, do)
+
+    # eager pass with kernel timers: the roofline of the dominant kernel
+    run(False)
+    al.synchronize()
+    al.reset_stats()
+    t0 = time.perf_counter()
+    run(False)
+    al.synchronize()
+    eager_ms = (time.perf_counter() - t0) * 1e3
+    rl, st = roofline_of(al, eager_ms)
+    reruns = int(st["rerun_pairs"])
+    use_graph = which == "c3"
+    run(use_graph)           # capture (C3)
+    al.synchronize()
+    steps = max(1, args.config_steps)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run(use_graph)
+    al.synchronize()
+    el = time.perf_counter() - t0
+    out = np.zeros(n, SCORE_DTYPE)
+    al.d2h(out, d_out)
+    cells = w.cells()
+    res = {
+        "workload": w.name, "sequences": len(w.offsets) - 1, "pairs": n,
+        "batch_pairs": batch, "graph_batches": use_graph,
+        "steps": steps, "ms_per_step": round(el * 1e3 / steps, 3),
+        "value": round(cells * steps / el / 1e9, 3), "unit": "GCUPS",
+        "pairs_per_s": round(n * steps / el, 1), "cells_per_step": cells,
+        "eager_ms_per_step": round(eager_ms, 3),
+        "int32_rerun_pairs": reruns,
+        "int32_rerun_rate": round(reruns / n, 8),
+        "dtype": "int16 (packed; int32 re-run on saturation)",
+        "roofline": rl,
+    }
+    al.device_free(d_pairs)
+    al.device_free(d_out)
+    al.close()
+    if not args.no_cpu:
+        go, ge = (-2, -11) if args.gap_model == "as_called" else (-11, -2)
+        idx, ref, cb = cpu_baseline_sw(w, pairs, go, ge, args.cpu_seconds,
+                                       host_cores())
+        ok = (np.array_equal(out["score"][idx], ref[0]) and
+              np.array_equal(out["r_end"][idx].astype(np.int32), ref[1]) and
+              np.array_equal(out["c_end"][idx].astype(np.int32), ref[2]))
+        res["cpu_baseline"] = cb
+        res["parity_on_cpu_sample"] = "bit-exact" if ok else "MISMATCH"
+    return res
+
+
+def c5_train_filter(args, cfg):
+    """Learned filter for the candidate stream, trained on GPU-aligner
+    labels of the candidates of a separate, smaller set from the same
+    generator (seed 205); tau calibrated on a held-out tenth for 99.5 %
+    recall with a one-sided 95 % margin.  Returns (params, mean, istd,
+    tau, info)."""
+    from pastis_amd import prefilter, synth
+    from pastis_amd.aligner import Aligner
+    t = time.time()
+    tw = synth.config5(scale=0.2 * args.c5_scale, seed=205)
+    tal = Aligner(cfg)
+    tal.load_seqs(tw.codes, tw.offsets)
+    cand = tal.kmer_candidates(k=6, symmetric="upper", ckthr=1)
+    rng = np.random.default_rng(7)
+    if len(cand) > 2_000_000:
+        cand = cand[np.sort(rng.choice(len(cand), 2_000_000, replace=False))]
+    y = prefilter.labels_of(tal.align(cand))
+    flt = prefilter.LearnedFilter(tal, seed=0)
+    f = flt.features(cand)
+    perm = rng.permutation(len(cand))
+    f, y = f[perm], y[perm]
+    ntr = len(y) * 9 // 10
+    t1 = time.time()
+    losses = flt.train(f[:ntr], y[:ntr], epochs=4, batch=4096, lr=2e-3)
+    t_train = time.time() - t1
+    tau = flt.calibrate(f[ntr:], y[ntr:], recall=0.995, z=1.645)
+    params, mean, istd = tal.filter_get_model()
+    tal.close()
+    info = {"train_pairs": int(ntr), "train_positives": int(y[:ntr].sum()),
+            "train_seconds": round(t_train, 3),
+            "final_loss": round(float(losses[-1]), 5),
+            "tau_logit": round(float(tau), 5),
+            "setup_seconds": round(time.time() - t, 1)}
+    log("C5 filter: %s" % info)
+    return params, mean, istd, tau, info
+
+
+def measure_c5(args):
+    """End-to-end candidate stream on one GPU (see the module docstring)."""
+    import ctypes as C
+    from pastis_amd import io, prefilter, synth
+    from pastis_amd._lib import PAIR_DTYPE, RESULT_DTYPE, TRIPLE_DTYPE
+    from pastis_amd.aligner import Aligner, config_for_cli
+    cfg = config_for_cli(-11, -2, 6, 49, 2, args.gap_model)
+    t = time.time()
+    w = synth.config5(scale=args.c5_scale)
+    nseq = len(w.offsets) - 1
+    log("C5: %d sequences (%d residues) generated in %.1f s" %
+        (nseq, len(w.codes), time.time() - t))
+    params, mean, istd, tau, finfo = c5_train_filter(args, cfg)
+    al = Aligner(cfg)
+    al.load_seqs(w.codes, w.offsets)
+    al.filter_set_model(params, mean, istd)
+    # candidate generation: column blocks, kept resident
+    kmer = dict(k=6, symmetric="upper", ckthr=1)
+    nblk = 4
+    bounds = [(nseq * i // nblk, nseq * (i + 1) // nblk) for i in range(nblk)]
+    al.synchronize()
+    t0 = time.perf_counter()
+    counts = [al.kmer_candidates_dev(None, 0, c_begin=a, c_end=b, **kmer)
+              for a, b in bounds]
+    gen_count_s = time.perf_counter() - t0
+    ncand = int(sum(counts))
+    d_all = al.device_alloc(max(ncand, 1) * PAIR_DTYPE.itemsize)
+    off = 0
+    t0 = time.perf_counter()
+    for (a, b), m in zip(bounds, counts):
+        dp = C.c_void_p(d_all.value + off * PAIR_DTYPE.itemsize)
+        al.kmer_candidates_dev(dp, m, c_begin=a, c_end=b, **kmer)
+        off += m
+    gen_s = gen_count_s + (time.perf_counter() - t0)
+    log("C5: %d candidates in %.2f s" % (ncand, gen_s))
+    batch = 10_000_000
+    b = min(batch, max(ncand, 1))
+    d_res = al.device_alloc(b * RESULT_DTYPE.itemsize)
+    d_kept = al.device_alloc(b * PAIR_DTYPE.itemsize)
+    d_idx = al.device_alloc(b * 4)
+    d_tri = al.device_alloc(max(ncand, 1) * TRIPLE_DTYPE.itemsize)
+    flags = np.zeros(ncand, np.uint16)
+    kept_pass = [0]
+
+    def stream(filtered, collect=False):
+        ntri = 0
+        for o in range(0, ncand, batch):
+            m = min(batch, ncand - o)
+            dp = C.c_void_p(d_all.value + o * PAIR_DTYPE.itemsize)
+            if filtered:
+                m = al.filter_select_dev(dp, m, float(tau), d_kept, d_idx)
+                dp = d_kept
+            if m == 0:
+                continue
+            al.align_dev(dp, m, d_res)
+            dt = C.c_void_p(d_tri.value + ntri * TRIPLE_DTYPE.itemsize)
+            ntri += io.triples_dev(al, dp, d_res, m, dt)
+            if collect:
+                r = np.zeros(m, RESULT_DTYPE)
+                al.d2h(r, d_res)
+                if filtered:
+                    kept_pass[0] += int(np.sum(prefilter.labels_of(r)))
+                else:
+                    flags[o:o + m] = r["flags"]
+        return ntri
+
+    # untimed passes: labels of every candidate, the filter's kept positives
+    ntri_plain = stream(False, collect=True)
+    ntri_filt = stream(True, collect=True)
+    npos = int(np.sum((flags & 1) != 0))
+    al.synchronize()
+    steps = max(1, args.config_steps)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        stream(False)
+    al.synchronize()
+    plain_s = (time.perf_counter() - t0) / steps
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        stream(True)
+    al.synchronize()
+    filt_s = (time.perf_counter() - t0) / steps
+    tri = np.zeros(ntri_filt, TRIPLE_DTYPE)
+    if ntri_filt:
+        al.d2h(tri, d_tri)
+    res = {
+        "workload": w.name, "sequences": nseq, "residues": int(len(w.codes)),
+        "kmer": "k=6, strict upper triangle, ckthr=1, pdefault alphabet",
+        "candidates": ncand, "batch_pairs": batch,
+        "candidate_generation_s": round(gen_s, 3),
+        "candidate_generation_pairs_per_s": round(ncand / max(gen_s, 1e-9), 1),
+        "passing_pairs": npos, "triples_plain": ntri_plain,
+        "triples_filtered": ntri_filt,
+        "plain": {"s_per_step": round(plain_s, 4),
+                  "pairs_per_s": round(ncand / plain_s, 1)},
+        "filter_exact": {"s_per_step": round(filt_s, 4),
+                         "pairs_per_s": round(ncand / filt_s, 1)},
+        "speedup_filter_vs_plain": round(plain_s / filt_s, 3),
+        "recall": round(kept_pass[0] / npos, 5) if npos else 1.0,
+        "end_to_end_pairs_per_s_plain": round(ncand / (gen_s + plain_s), 1),
+        "end_to_end_pairs_per_s_filter":
+            round(ncand / (gen_s + filt_s), 1),
+        "filter": finfo,
+    }
+    pairs = None
+    if not args.no_cpu and ncand:
+        pairs = np.zeros(min(ncand, 2_000_000), PAIR_DTYPE)
+        rng = np.random.default_rng(11)
+        # the CPU sample is drawn from the resident candidates
+        idx_all = np.sort(rng.choice(ncand, len(pairs), replace=False))
+        full = np.zeros(ncand, PAIR_DTYPE)
+        al.d2h(full, d_all)
+        pairs = full[idx_all]
+    for x in (d_all, d_res, d_kept, d_idx, d_tri):
+        al.device_free(x)
+    al.close()
+    if pairs is not None:
+        from oracle import oracle as orc
+        g = ({"gap_open": -2, "gap_ext": -11, "xdrop_gap": -11}
+             if args.gap_model == "as_called" else
+             {"gap_open": -11, "gap_ext": -2, "xdrop_gap": -2})
+        p = orc.params(g["gap_open"], g["gap_ext"], g["xdrop_gap"], 49, 6, 2,
+                       0.7, 30)
+        wk = synth.Workload(w.name, w.codes, w.offsets, pairs, w.family, {})
+        _, ref, cb = cpu_baseline_xdrop(wk, pairs, p, args.cpu_seconds,
+                                        host_cores())
+        cb["unit"] = "candidate pairs/s (X-drop + statistics)"
+        cb["value"] = cb.pop("pairs_per_s")
+        res["cpu_baseline"] = cb
+    log("C5: %s" % {k: v for k, v in res.items() if k != "filter"})
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -260,6 +534,13 @@ def main():
                     help="keep the default launch shapes")
     ap.add_argument("--no-filter", action="store_true",
                     help="skip the learned-filter vs plain comparison")
+    ap.add_argument("--configs", default="c1,c3,c5",
+                    help="extra configurations measured after the headline "
+                         "(comma list of c1, c3, c5, or none)")
+    ap.add_argument("--c3-scale", type=float, default=1.0)
+    ap.add_argument("--c5-scale", type=float, default=1.0)
+    ap.add_argument("--config-steps", type=int, default=2,
+                    help="timed steps of each extra configuration")
     args = ap.parse_args()
     if args.copy_probe > 0:
         import torch
@@ -413,6 +694,25 @@ def main():
         line["parity_on_cpu_sample"] = "bit-exact" if ok else "MISMATCH"
     al.device_free(d_pairs)
     al.device_free(d_out)
+    al.close()
+    extra = [c.strip() for c in args.configs.split(",")
+             if c.strip() and c.strip() != "none"]
+    if extra:
+        line["configs"] = {}
+    for c in extra:
+        t = time.time()
+        try:
+            if c == "c1":
+                line["configs"]["C1"] = measure_sw_config(args, "c1")
+            elif c == "c3":
+                line["configs"]["C3"] = measure_sw_config(args, "c3")
+            elif c == "c5":
+                line["configs"]["C5"] = measure_c5(args)
+            else:
+                raise ValueError("unknown configuration " + c)
+        except Exception as e:   # keep the headline line if one fails
+            line["configs"][c.upper()] = {"error": repr(e)}
+        log("config %s done in %.1f s" % (c, time.time() - t))
     print(json.dumps(line), flush=True)
 
 

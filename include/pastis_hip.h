@@ -194,11 +194,13 @@ int pa_sw_scores_dev(struct pa_ctx *ctx, const pa_pair *d_pairs, uint64_t n,
 
 /* pa_sw_scores_dev as a replayed HIP graph: the first call with a given
  * (d_pairs, n, d_out, stream) captures the batch's kernels into a graph,
- * later calls replay it.  The pairs' contents may change between calls.  A
- * different (d_pairs, n, d_out, stream) recaptures, and so does a call after
- * anything that moved or resized the context's workspace or changed its
- * sequence sets or tuning (a larger eager batch, pa_load_seqs,
- * pa_set_tuning), so eager calls may be mixed with replays.
+ * later calls replay it; up to 8 such graphs are kept (least recently used
+ * dropped), so a loop over the batches of a resident pair array replays one
+ * graph per batch.  The pairs' contents may change between calls.  Every
+ * graph is recaptured after anything that moved or resized the context's
+ * workspace or changed its sequence sets or tuning (a larger eager batch,
+ * pa_load_seqs, pa_set_tuning), so eager calls may be mixed with replays.
+ * The graphs share the context's workspace: replay them on one stream.
  * Replaces the per-batch launches behind pw_aln_batch's loop
  * (src/DistPWRunner.cpp:116-131) for score-only batches of equal size. */
 int pa_sw_scores_graph(struct pa_ctx *ctx, const pa_pair *d_pairs, uint64_t n,

@@ -283,7 +283,8 @@ template <int L> __device__ inline uint64_t group_max_u64(uint64_t v)
 }
 
 // L lanes per task: 64 (one task per wave) or 16 (four tasks per wave, each
-// on one 16-lane DPP row; single-pass tasks only).
+// on one 16-lane DPP row, in lockstep; the wave runs as many passes as its
+// longest task needs, and each slot keeps its own boundary row).
 template <int R, int L, bool LOCAL, bool TRACE, bool I16>
 __global__ __launch_bounds__(64) void k_dp(DpArgs a)
 {
@@ -321,7 +322,9 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
     const V EXTE = O::splat(TRACE ? ext + 2 : ext);
     // F value of "no cell": NEG, tagged like every carried F when traced
     constexpr int NEGF = TRACE ? NEG + 1 : NEG;
-    int32_t PA_GLOBAL *bnd = glb(a.bnd) + (size_t)blockIdx.x * a.bnd_stride;
+    // boundary row of this task slot (multi-pass tasks)
+    int32_t PA_GLOBAL *bnd =
+        glb(a.bnd) + ((size_t)blockIdx.x * G + (size_t)grp) * a.bnd_stride;
 
     // tasks claimed by this wave and not yet started: [cbase, cend)
     uint32_t cbase = 0, cend = 0;
@@ -460,7 +463,7 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
             const int cap_half = (qrow % (2 * R)) / R;
             const int cap_r = qrow % R;
             const int cap_step = tlen + 2 * cap_lane + cap_half;
-            const bool last_pass = p == npass - 1;
+            const bool last_pass = p == npass_own - 1;   // this slot's
 
             // trace record of (step 1, this lane); the next step's record
             // follows it, except after the last step of a segment, which
@@ -468,7 +471,7 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
             const int t_first = 2 * gl + 1;
             const int t_last = active ? tlen + 1 + 2 * gl : 0;
             uint8_t PA_GLOBAL *trp =
-                TRACE && active ? tr + trace_rec(qlen, tlen, R, p, 1, gl) * R
+                TRACE && active ? tr + trace_rec(qlen, tlen, R, L, p, 1, gl) * R
                                 : tr;
             const uint32_t pass_lanes =
                 TRACE ? (p + 1 < npass_own ? (uint32_t)L : (uint32_t)nact) : 1u;
@@ -494,16 +497,18 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
                        (res_at(c + 2) << 16) | (res_at(c + 3) << 24);
             };
             int rc_cur = res_blk(1), rc_nxt = res_blk(65);
+            // Boundary row of a later pass: blocks of L columns, lane gl of
+            // a slot holding column base + gl; the slot's first lane takes
+            // column jn with readlane (L = 64) or a shuffle within the slot.
             int bh_cur = 0, bf_cur = NEGF, bh_nxt = 0, bf_nxt = NEGF;
             if (p > 0) {
-                // boundary block k holds columns 64k .. 64k+63
-                if (lane <= tlen) {   // 64-lane tasks only
-                    bh_cur = bnd[2 * lane];
-                    bf_cur = bnd[2 * lane + 1];
+                if (gl <= tlen) {
+                    bh_cur = bnd[2 * gl];
+                    bf_cur = bnd[2 * gl + 1];
                 }
-                if (64 + lane <= tlen) {
-                    bh_nxt = bnd[2 * (64 + lane)];
-                    bf_nxt = bnd[2 * (64 + lane) + 1];
+                if (L + gl <= tlen) {
+                    bh_nxt = bnd[2 * (L + gl)];
+                    bf_nxt = bnd[2 * (L + gl) + 1];
                 }
             }
             // Column residues of step tn from those of step tn-1: lane 0
@@ -560,17 +565,24 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
                         if constexpr (FIRST) {
                             nbh = LOCAL ? 0 : open + (jn - 1) * gmax;
                         } else {
-                            const int bi = jn & 63;
-                            if (bi == 0) {
+                            const int bi = jn & (L - 1);
+                            if (bi == 0) {   // wave-uniform
                                 bh_cur = bh_nxt;
                                 bf_cur = bf_nxt;
-                                const int jb = jn + 64 + lane;
+                                const int jb = jn + L + gl;
                                 bh_nxt = jb <= tlen ? bnd[2 * jb] : 0;
                                 bf_nxt = jb <= tlen ? bnd[2 * jb + 1] : NEGF;
                             }
-                            if (jn <= tlen) {
-                                nbh = __builtin_amdgcn_readlane(bh_cur, bi);
-                                nbf = __builtin_amdgcn_readlane(bf_cur, bi);
+                            if (L == 64) {
+                                if (jn <= tlen) {
+                                    nbh = __builtin_amdgcn_readlane(bh_cur, bi);
+                                    nbf = __builtin_amdgcn_readlane(bf_cur, bi);
+                                }
+                            } else {
+                                const int xh = __shfl(bh_cur, grp * L + bi);
+                                const int xf = __shfl(bf_cur, grp * L + bi);
+                                nbh = jn <= tlen ? xh : nbh;
+                                nbf = jn <= tlen ? xf : nbf;
                             }
                         }
                     }
@@ -658,7 +670,8 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
                                 if (r == cap_r)
                                     gscore = cap_half ? O::hi(H[r]) : O::lo(H[r]);
                         }
-                        if (p + 1 < npass && gl == L - 1 && jhi >= 0 && jhi <= tlen) {
+                        if (p + 1 < npass_own && gl == L - 1 && jhi >= 0 &&
+                            jhi <= tlen) {
                             bnd[2 * jhi] = O::hi(outH);
                             bnd[2 * jhi + 1] = O::hi(outF);
                         }
@@ -723,8 +736,8 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
                 }
             }
         } else {
-            // the capturing lane of the last pass holds H(qlen, tlen)
-            const int qrow = qlen - 1 - (npass - 1) * ROWS;
+            // the capturing lane of the slot's last pass holds H(qlen, tlen)
+            const int qrow = qlen - 1 - (npass_own - 1) * ROWS;
             const int cl = qrow / (2 * R);
             int g = __shfl(gscore, grp * L + (cl & (L - 1)));
             o.score = g >> SH;
@@ -757,9 +770,24 @@ namespace {
 // divides by constants.  Global traces mark a diagonal step between equal
 // residues with tag 3 (restart does not occur there), so LOCAL = false needs
 // no residue loads.
-template <int R, bool LOCAL>
-__global__ __launch_bounds__(256) void k_traceback(TbArgs a)
+//
+// The walk reads the trace one lane run at a time: the kTraceSeg records of
+// one lane in one segment (8R contiguous bytes, 8R-aligned) are loaded with
+// wide loads into the thread's slot in LDS when the walk enters them, and
+// the codes are read from there.  Diagonal and horizontal moves step back
+// one record inside the run, vertical moves stay in the record, so a run
+// serves several steps; reading single bytes from global memory instead
+// lets the lines of thousands of concurrent walks evict each other before
+// their next byte is read.
+constexpr int kTbThreads = 256;
+
+template <int R, int L, bool LOCAL>
+__global__ __launch_bounds__(kTbThreads) void k_traceback(TbArgs a)
 {
+    constexpr int RUNW = 2 * R;   // 32-bit words of one lane run (8R bytes)
+    // one run per thread; the odd stride spreads threads over the banks
+    __shared__ uint32_t stage[kTbThreads * (RUNW + 1)];
+    uint32_t *run = stage + threadIdx.x * (RUNW + 1);
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= *a.count)
         return;
@@ -769,21 +797,22 @@ __global__ __launch_bounds__(256) void k_traceback(TbArgs a)
     const uint8_t PA_GLOBAL *qs = glb_addr<const uint8_t>(T.q_off);
     const uint8_t PA_GLOBAL *ts = glb_addr<const uint8_t>(T.t_off);
     const int tlen = (int)T.tlen;
-    const uint8_t PA_GLOBAL *tr = glb(a.trace) + glb(a.trace_off)[tid];
+    const uint8_t *trg = a.trace + glb(a.trace_off)[tid];
     // Position of row i-1 in the trace layout: pass p, lane l, half h, row r
     // of the half; kept up to date as the walk moves up, so a step costs a
     // few adds.  Byte offsets within one task's trace fit in 32 bits.
-    constexpr int ROWS = 128 * R;
+    constexpr int ROWS = 2 * R * L;
     constexpr uint32_t RUN = kTraceSeg * R;   // one lane's run of a segment
     const uint32_t qlen = T.qlen;
     const uint32_t full = qlen ? (qlen - 1) / ROWS : 0;
-    const uint32_t last_lanes = qlen ? trace_last_lanes(qlen, R) : 0;
-    const uint32_t pass_bytes = trace_seg_steps((uint32_t)tlen + 127u) * 64u * R;
+    const uint32_t last_lanes = qlen ? trace_last_lanes(qlen, R, L) : 0;
+    const uint32_t pass_bytes =
+        trace_seg_steps((uint32_t)tlen + 2u * L - 1u) * (uint32_t)L * R;
     int p = 0, l = 0, h = 0, r = 0;
     uint32_t pbase = 0, seg_bytes = 0;   // of pass p
     auto set_pass = [&]() {
         pbase = (uint32_t)p * pass_bytes;
-        seg_bytes = ((uint32_t)p < full ? 64u : last_lanes) * RUN;
+        seg_bytes = ((uint32_t)p < full ? (uint32_t)L : last_lanes) * RUN;
     };
     int i = o.q_end, j = o.t_end;
     if (i > 0) {
@@ -801,13 +830,14 @@ __global__ __launch_bounds__(256) void k_traceback(TbArgs a)
             if (--h < 0) {
                 h = 1;
                 if (--l < 0) {
-                    l = 63;
+                    l = L - 1;
                     if (--p >= 0)
                         set_pass();
                 }
             }
         }
     };
+    uint32_t cur_run = 0xffffffffu;   // byte offset of the staged run
     int matches = 0, cols = 0, runs = 0;
     int last = 0;   // 1 diagonal, 2 vertical, 3 horizontal
     int state = 0;  // 0 H, 1 E (horizontal), 2 F (vertical)
@@ -833,10 +863,25 @@ __global__ __launch_bounds__(256) void k_traceback(TbArgs a)
             // record of (step t, lane l); row r, half h of a record sits in
             // byte 4*(r/4) + 2*(r%2) + h, high nibble when (r/2) is odd
             const uint32_t t1 = (uint32_t)(j + 2 * l + h) - 1u;
-            const uint32_t off = pbase + (t1 / kTraceSeg) * seg_bytes +
-                                 (uint32_t)l * RUN + (t1 % kTraceSeg) * R +
-                                 4u * (r >> 2) + 2u * (r & 1) + (uint32_t)h;
-            const uint32_t byte = tr[off];
+            const uint32_t run_off =
+                pbase + (t1 / kTraceSeg) * seg_bytes + (uint32_t)l * RUN;
+            if (run_off != cur_run) {
+                cur_run = run_off;
+                // generic pointer: vector-struct loads from the global
+                // address space do not compile
+                const uint4 *src = (const uint4 *)(trg + run_off);
+#pragma unroll
+                for (int q = 0; q < RUNW / 4; ++q) {
+                    const uint4 x = src[q];
+                    run[4 * q + 0] = x.x;
+                    run[4 * q + 1] = x.y;
+                    run[4 * q + 2] = x.z;
+                    run[4 * q + 3] = x.w;
+                }
+            }
+            const uint32_t bi = (t1 % kTraceSeg) * R + 4u * (r >> 2) +
+                                2u * (r & 1) + (uint32_t)h;
+            const uint32_t byte = (run[bi >> 2] >> (8u * (bi & 3u))) & 0xffu;
             const int c = (int)((r & 2) ? (byte >> 4) : (byte & 15u));
             if (state == 0) {
                 const int src = c & 3;
@@ -928,16 +973,34 @@ hipError_t launch_c(bool local, bool trace, bool i16, const DpArgs &a,
                : launch_one<R, L, false, true, false>(a, grid, s);
 }
 
+// kernel shape of each DP class (class_rows / class_lanes)
+template <int CLS> struct Shape {
+    static constexpr int R = class_rows(CLS), L = class_lanes(CLS);
+};
+
+template <int CLS>
+hipError_t launch_traceback_c(const TbArgs &a, int grid, hipStream_t s)
+{
+    constexpr int R = Shape<CLS>::R, L = Shape<CLS>::L;
+    if (a.local)
+        hipLaunchKernelGGL((k_traceback<R, L, true>), dim3(grid),
+                           dim3(kTbThreads), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_traceback<R, L, false>), dim3(grid),
+                           dim3(kTbThreads), 0, s, a);
+    return hipGetLastError();
+}
+
 }  // namespace
 
 hipError_t launch_dp(int cls, bool local, bool trace, bool i16,
                      const DpArgs &a, int grid, hipStream_t s)
 {
     switch (cls) {
-    case 0: return launch_c<4, 16>(local, trace, i16, a, grid, s);
-    case 1: return launch_c<8, 16>(local, trace, i16, a, grid, s);
-    case 2: return launch_c<4, 64>(local, trace, i16, a, grid, s);
-    case 3: return launch_c<8, 64>(local, trace, i16, a, grid, s);
+    case 0: return launch_c<Shape<0>::R, Shape<0>::L>(local, trace, i16, a, grid, s);
+    case 1: return launch_c<Shape<1>::R, Shape<1>::L>(local, trace, i16, a, grid, s);
+    case 2: return launch_c<Shape<2>::R, Shape<2>::L>(local, trace, i16, a, grid, s);
+    case 3: return launch_c<Shape<3>::R, Shape<3>::L>(local, trace, i16, a, grid, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -949,31 +1012,23 @@ int dp_occupancy(int cls, bool local, bool trace, bool i16)
     if (cls < 0 || cls > 3)
         return 1;
     if (!cache[cls][vi])
-        cache[cls][vi] = cls == 0   ? occupancy_c<4, 16>(local, trace, i16)
-                         : cls == 1 ? occupancy_c<8, 16>(local, trace, i16)
-                         : cls == 2 ? occupancy_c<4, 64>(local, trace, i16)
-                                    : occupancy_c<8, 64>(local, trace, i16);
+        cache[cls][vi] =
+            cls == 0   ? occupancy_c<Shape<0>::R, Shape<0>::L>(local, trace, i16)
+            : cls == 1 ? occupancy_c<Shape<1>::R, Shape<1>::L>(local, trace, i16)
+            : cls == 2 ? occupancy_c<Shape<2>::R, Shape<2>::L>(local, trace, i16)
+                       : occupancy_c<Shape<3>::R, Shape<3>::L>(local, trace, i16);
     return cache[cls][vi];
 }
 
-hipError_t launch_traceback(const TbArgs &a, int rows_per_lane, int grid,
-                            hipStream_t s)
+hipError_t launch_traceback(const TbArgs &a, int cls, int grid, hipStream_t s)
 {
-    if (rows_per_lane == 4 && a.local)
-        hipLaunchKernelGGL((k_traceback<4, true>), dim3(grid), dim3(256), 0, s,
-                           a);
-    else if (rows_per_lane == 4)
-        hipLaunchKernelGGL((k_traceback<4, false>), dim3(grid), dim3(256), 0,
-                           s, a);
-    else if (rows_per_lane == 8 && a.local)
-        hipLaunchKernelGGL((k_traceback<8, true>), dim3(grid), dim3(256), 0, s,
-                           a);
-    else if (rows_per_lane == 8)
-        hipLaunchKernelGGL((k_traceback<8, false>), dim3(grid), dim3(256), 0,
-                           s, a);
-    else
-        return hipErrorInvalidValue;
-    return hipGetLastError();
+    switch (cls) {
+    case 0: return launch_traceback_c<0>(a, grid, s);
+    case 1: return launch_traceback_c<1>(a, grid, s);
+    case 2: return launch_traceback_c<2>(a, grid, s);
+    case 3: return launch_traceback_c<3>(a, grid, s);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace pa

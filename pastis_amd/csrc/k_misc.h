@@ -6,13 +6,17 @@
 namespace pa {
 
 // Bins: 2 * DP class (dp_class) + (1 for the int32 kernels).  Inside a bin
-// the tasks of the 16-lane classes are ordered by kBuckets buckets of their
-// step count, so that the four tasks a wave runs together are alike.
+// the tasks are ordered by kBuckets log-spaced buckets of their step count,
+// longest first: the tasks a wave runs together are alike, and the
+// persistent grid ends on short tasks.
 constexpr int kClasses = 4;
 constexpr int kBins = 2 * kClasses;
-constexpr int kBuckets = 16;
+constexpr int kBuckets = 64;
 
 hipError_t fill_u32(uint32_t *p, uint32_t v, uint32_t n, hipStream_t s);
+// *dst += *src (one thread; a device-side counter)
+hipError_t accumulate(const uint32_t *src, unsigned long long *dst,
+                      hipStream_t s);
 // Pairs naming sequences outside the nr row / nc column sequences become
 // empty tasks (and X-drop tasks whose seed does not fit are flagged
 // kXdInvalid), so no kernel reads outside the sequence store.

@@ -32,6 +32,12 @@ inline int stride_blocks(uint64_t n)
     return (int)(b < 2048 ? (b > 0 ? b : 1) : 2048);
 }
 
+__global__ void k_accumulate(const uint32_t *src, unsigned long long *dst)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0 && *src)
+        atomicAdd(dst, (unsigned long long)*src);
+}
+
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, uint32_t n)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -155,8 +161,18 @@ __device__ inline bool global_i16_ok(uint32_t q, uint32_t t, uint32_t flags,
     return (4 * ub + 3 < 32760 || (flags & kTaskBounded)) && 4 * lb > -31000;
 }
 
-// Bin and bucket of a task (-1: empty task).  The bucket of a 16-lane class
-// task is its step count / 32 (capped); other classes use bucket 0.
+// Wavefront steps of a task in its class's kernel, all passes.
+__device__ inline uint32_t task_steps(uint32_t qlen, uint32_t tlen)
+{
+    const int R = rows_per_lane(qlen), L = lanes_per_task(qlen);
+    const uint32_t rows = 2u * R * L;
+    const uint32_t npass = (qlen + rows - 1) / rows;
+    const uint32_t nl = trace_last_lanes(qlen, R, L);
+    return (npass - 1) * (tlen + 2u * L - 1u) + tlen + 2u * (nl - 1u) + 1u;
+}
+
+// Bin and bucket of a task (-1: empty task).  Buckets are 1/8 of an octave
+// of the step count wide (16 .. 2^23 steps), the longest in bucket 0.
 __device__ inline int bin_key(const DpTask &t, int local, int maxs, int open,
                               int ext)
 {
@@ -167,13 +183,9 @@ __device__ inline int bin_key(const DpTask &t, int local, int maxs, int open,
         local ? true
               : global_i16_ok(t.qlen, t.tlen, t.flags, maxs, open, ext);
     const int bin = 2 * rc + (i16 ? 0 : 1);
-    int bucket = 0;
-    if (rc <= 1) {
-        const int R = rows_per_lane(t.qlen);
-        const int nact = ((int)t.qlen + 2 * R - 1) / (2 * R);
-        const int steps = (int)t.tlen + 2 * (nact - 1) + 1;
-        bucket = min(kBuckets - 1, steps >> 5);
-    }
+    const float lg = __log2f((float)task_steps(t.qlen, t.tlen) * (1.0f / 16.f));
+    const int b = (int)(8.0f * (lg > 0.f ? lg : 0.f));
+    const int bucket = kBuckets - 1 - (b < kBuckets - 1 ? b : kBuckets - 1);
     return bin * kBuckets + bucket;
 }
 
@@ -258,7 +270,8 @@ __global__ void k_trace_sizes(const DpTask *__restrict__ tasks, uint64_t n,
     if (i < n) {
         const DpTask t = tasks[i];
         sizes[i] = (t.qlen && t.tlen)
-                       ? trace_bytes(t.qlen, t.tlen, rows_per_lane(t.qlen))
+                       ? trace_bytes(t.qlen, t.tlen, rows_per_lane(t.qlen),
+                                     lanes_per_task(t.qlen))
                        : 0;
     }
 }
@@ -435,6 +448,13 @@ hipError_t fill_u32(uint32_t *p, uint32_t v, uint32_t n, hipStream_t s)
     if (!n)
         return hipSuccess;
     hipLaunchKernelGGL(k_fill_u32, dim3(blocks(n, 256)), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t accumulate(const uint32_t *src, unsigned long long *dst,
+                      hipStream_t s)
+{
+    hipLaunchKernelGGL(k_accumulate, dim3(1), dim3(64), 0, s, src, dst);
     return hipGetLastError();
 }
 

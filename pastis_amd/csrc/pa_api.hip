@@ -89,6 +89,7 @@ enum {
 };
 
 constexpr int kMaxWgPerCu = 32;     // one-wave workgroups per CU (8 per SIMD)
+constexpr int kSlotsPerWg = 4;      // task slots of a 16-lane-group wave
 constexpr int kXdWideLanes = 65536;  // lanes of the wide (global scratch) re-run
 
 }  // namespace
@@ -207,10 +208,11 @@ int xd_grid(pa_ctx *c, uint64_t nitems)
 
 int ensure_bnd(pa_ctx *c)
 {
-    // one boundary row per persistent workgroup the DP grids can hold
+    // one boundary row per task slot of each persistent workgroup the DP
+    // grids can hold (64 / 16 slots per one-wave workgroup)
     const int wg = c->wg_cap > 0 ? std::min(c->wg_cap, kMaxWgPerCu)
                                  : kMaxWgPerCu;
-    const size_t need = (size_t)c->ncu * wg * 2u *
+    const size_t need = (size_t)c->ncu * wg * kSlotsPerWg * 2u *
                         (size_t)(c->max_clen + 2) * sizeof(int32_t);
     return grow(c, c->bnd, need);
 }
@@ -334,8 +336,8 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
                 const int bin = 2 * cls + w;
                 tb.list = lists + (size_t)bin * nt;
                 tb.count = ctr + C_BIN + bin;
-                HIPCHK(c, launch_traceback(tb, cls == 0 || cls == 2 ? 4 : 8,
-                                           (int)((b - a + 255) / 256), ts));
+                HIPCHK(c, launch_traceback(tb, cls, (int)((b - a + 255) / 256),
+                                           ts));
                 c->stats.launches++;
             }
             return PA_OK;
@@ -376,7 +378,12 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
                     }
                     HIPCHK(c, mark_flags(args.list, args.count, b - a,
                                          P<uint8_t>(c->tflags), 1, s));
-                    c->stats.launches += 2;
+                    // tasks re-run in int32 (pa_stats.rerun_pairs)
+                    HIPCHK(c, accumulate(args.count,
+                                         (unsigned long long *)P<uint64_t>(
+                                             c->cells) + 2,
+                                         s));
+                    c->stats.launches += 3;
                 }
                 if (split) {
                     // this class's trace is complete: trace it back on st2
@@ -846,10 +853,11 @@ void pa_destroy(pa_ctx *c)
     if (c->st)
         hipStreamSynchronize(c->st);
     resolve_timers(c);
-    if (c->gexec)
-        hipGraphExecDestroy(c->gexec);
-    if (c->graph)
-        hipGraphDestroy(c->graph);
+    for (auto &e : c->graphs) {
+        hipGraphExecDestroy(e.x);
+        hipGraphDestroy(e.g);
+    }
+    c->graphs.clear();
     if (c->shared_cols) {
         c->ccodes = Buf();
         c->coffs = Buf();
@@ -1061,17 +1069,36 @@ int pa_sw_scores_graph(pa_ctx *c, const pa_pair *d_pairs, uint64_t n,
         return fail(c, PA_ERR_LIMIT, "more than 2^32 pairs");
     hipSetDevice(c->device);
     hipStream_t s = pick(c, stream);
-    if (!c->gexec || c->g_pairs != d_pairs || c->g_n != n || c->g_out != d_out ||
-        c->g_stream != s || c->g_gen != c->gen) {
-        if (c->gexec)
-            hipGraphExecDestroy(c->gexec);
-        if (c->graph)
-            hipGraphDestroy(c->graph);
-        c->gexec = nullptr;
-        c->graph = nullptr;
-        // every buffer the batch uses exists before the capture starts
+    // graphs of an older workspace generation hold freed addresses
+    auto &G = c->graphs;
+    for (size_t i = 0; i < G.size();) {
+        if (G[i].gen != c->gen) {
+            hipGraphExecDestroy(G[i].x);
+            hipGraphDestroy(G[i].g);
+            G.erase(G.begin() + (long)i);
+        } else {
+            ++i;
+        }
+    }
+    pa_ctx::GraphEntry *hit = nullptr;
+    for (auto &e : G)
+        if (e.pairs == d_pairs && e.n == n && e.out == d_out && e.s == s)
+            hit = &e;
+    if (!hit) {
+        // every buffer the batch uses exists before the capture starts; a
+        // reallocation here invalidates the other graphs
         if ((rc = ensure_common(c, n)) || (rc = ensure_bnd(c)))
             return rc;
+        for (size_t i = 0; i < G.size();) {
+            if (G[i].gen != c->gen || (G.size() >= (size_t)pa_ctx::kGraphs &&
+                                       i == 0)) {
+                hipGraphExecDestroy(G[i].x);
+                hipGraphDestroy(G[i].g);
+                G.erase(G.begin() + (long)i);
+            } else {
+                ++i;
+            }
+        }
         HIPCHK(c, hipStreamSynchronize(s));
         HIPCHK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         c->capturing = true;
@@ -1094,16 +1121,29 @@ int pa_sw_scores_graph(pa_ctx *c, const pa_pair *d_pairs, uint64_t n,
                 hipGraphDestroy(g);
             return fail(c, rc ? rc : PA_ERR_HIP, "graph capture failed");
         }
-        c->graph = g;
-        HIPCHK(c, hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
-        c->g_pairs = d_pairs;
-        c->g_n = n;
-        c->g_out = d_out;
-        c->g_stream = s;
-        c->g_gen = c->gen;
+        pa_ctx::GraphEntry ent;
+        ent.g = g;
+        if (hipGraphInstantiate(&ent.x, g, nullptr, nullptr, 0) !=
+            hipSuccess) {
+            hipGraphDestroy(g);
+            return fail(c, PA_ERR_HIP, "graph instantiation failed");
+        }
+        ent.pairs = d_pairs;
+        ent.n = n;
+        ent.out = d_out;
+        ent.s = s;
+        ent.gen = c->gen;
+        G.push_back(ent);
+        hit = &G.back();
         c->stats.graph_captures++;
     }
-    HIPCHK(c, hipGraphLaunch(c->gexec, s));
+    hit->used = ++c->graph_tick;
+    HIPCHK(c, hipGraphLaunch(hit->x, s));
+    // least recently used first, so that eviction takes entry 0
+    std::sort(G.begin(), G.end(),
+              [](const pa_ctx::GraphEntry &a, const pa_ctx::GraphEntry &b) {
+                  return a.used < b.used;
+              });
     c->stats.pairs += n;
     return PA_OK;
 }
@@ -1249,10 +1289,11 @@ int pa_get_stats(pa_ctx *c, pa_stats *out)
     HIPCHK(c, hipStreamSynchronize(c->st));
     resolve_timers(c);
     if (c->cells.p) {
-        uint64_t cells[2] = {0, 0};
-        HIPCHK(c, hipMemcpy(cells, c->cells.p, 16, hipMemcpyDeviceToHost));
+        uint64_t cells[3] = {0, 0, 0};
+        HIPCHK(c, hipMemcpy(cells, c->cells.p, 24, hipMemcpyDeviceToHost));
         c->stats.dp_cells = cells[0];
         c->stats.xdrop_cells = cells[1];
+        c->stats.rerun_pairs = cells[2];
     }
     *out = c->stats;
     return PA_OK;

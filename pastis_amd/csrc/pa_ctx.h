@@ -72,17 +72,26 @@ struct pa_ctx {
     pa::FilterState *flt = nullptr;
     pa::KmerState *kmer = nullptr;
     int ncu = 256;                // compute units of the device
-    // hipGraph of one score-only SW batch (pa_sw_scores_graph)
+    // hipGraphs of score-only SW batches (pa_sw_scores_graph), one per
+    // (pairs, n, out, stream), least recently used dropped beyond kGraphs.
+    // They share the workspace, so they replay one after another on a
+    // stream (as the batches of a pw_aln_batch loop do).
     bool capturing = false;       // no event timers while a stream captures
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    const void *g_pairs = nullptr, *g_out = nullptr;
-    uint64_t g_n = 0;
-    hipStream_t g_stream = nullptr;
-    // Workspace generation: bumped whenever a buffer the graph may have
+    struct GraphEntry {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t x = nullptr;
+        const void *pairs = nullptr, *out = nullptr;
+        uint64_t n = 0;
+        hipStream_t s = nullptr;
+        uint64_t gen = 0, used = 0;
+    };
+    static constexpr int kGraphs = 8;
+    std::vector<GraphEntry> graphs;
+    uint64_t graph_tick = 0;
+    // Workspace generation: bumped whenever a buffer a graph may have
     // captured is reallocated (grow), the sequence sets change or the launch
     // shapes change; a graph captured at another generation is recaptured.
-    uint64_t gen = 0, g_gen = 0;
+    uint64_t gen = 0;
     pa_tuning tune{0, 0, 16, 0};
     // No event timers (pa_sw_scores_noalloc: event objects are allocated).
     bool quiet = false;

@@ -111,20 +111,48 @@ struct TbArgs {
     int local;
 };
 
-// Trace layout of a task traced with R rows per lane: pass after pass; a
-// pass holds one record of R bytes (2R 4-bit codes) per (step, active lane).
-// Full passes have 64 active lanes and tlen + 127 steps; the last pass has
-// nact lanes and tlen + 2*(nact-1) + 1 steps.  Steps are grouped in segments
-// of kTraceSeg: a segment holds, lane after lane, the kTraceSeg consecutive
-// records of each lane, so a traceback that walks back along a lane reads
-// neighbouring bytes (8R contiguous bytes per lane and segment) while one
-// step of the DP still writes 64 records in a few cache lines.  A pass is
-// padded to whole segments.
+// DP kernel class of a task by its row count (the row sequence of the task
+// runs down the lanes) and the kernel shape of each class: L lanes per task
+// (G = 64 / L tasks per wave, one per L-lane group) and R rows per lane per
+// half, so a pass covers 2RL rows and longer tasks take several passes
+// linked by a boundary row:
+//   0: qlen <= 128: 16 lanes x 4 rows, 128 rows per pass (one pass);
+//   1: qlen <= 256: 16 lanes x 8 rows, 256 rows per pass (one pass);
+//   2: qlen <= 512: 16 lanes x 4 rows, 128 rows per pass (2-4 passes);
+//   3: longer:      16 lanes x 4 rows, 128 rows per pass.
+// Narrow groups keep more of the wave busy than 64-lane tasks do: the
+// wavefront's fill and drain take 2L steps per pass and the last pass
+// rounds the rows up to 2RL.
+__host__ __device__ inline int dp_class(uint32_t qlen)
+{
+    return qlen <= 128 ? 0 : (qlen <= 256 ? 1 : (qlen <= 512 ? 2 : 3));
+}
+__host__ __device__ constexpr int class_rows(int cls) { return cls == 1 ? 8 : 4; }
+__host__ __device__ constexpr int class_lanes(int) { return 16; }
+__host__ __device__ inline int rows_per_lane(uint32_t qlen)
+{
+    return class_rows(dp_class(qlen));
+}
+__host__ __device__ inline int lanes_per_task(uint32_t qlen)
+{
+    return class_lanes(dp_class(qlen));
+}
+
+// Trace layout of a task traced with R rows per lane and L lanes: pass after
+// pass; a pass holds one record of R bytes (2R 4-bit codes) per (step,
+// active lane).  Full passes have L active lanes and tlen + 2L - 1 steps;
+// the last pass has nact lanes and tlen + 2*(nact-1) + 1 steps.  Steps are
+// grouped in segments of kTraceSeg: a segment holds, lane after lane, the
+// kTraceSeg consecutive records of each lane, so a traceback that walks
+// back along a lane reads neighbouring bytes (8R contiguous bytes per lane
+// and segment) while one step of the DP still writes L records in a few
+// cache lines.  A pass is padded to whole segments.
 constexpr uint32_t kTraceSeg = 8;
 
-__host__ __device__ inline uint32_t trace_last_lanes(uint32_t qlen, int R)
+__host__ __device__ inline uint32_t trace_last_lanes(uint32_t qlen, int R,
+                                                     int L)
 {
-    const uint32_t rows = 128u * (uint32_t)R;
+    const uint32_t rows = 2u * (uint32_t)R * (uint32_t)L;
     const uint32_t rem = qlen - ((qlen - 1) / rows) * rows;
     return (rem + 2u * R - 1u) / (2u * R);
 }
@@ -136,46 +164,27 @@ __host__ __device__ inline uint32_t trace_seg_steps(uint32_t steps)
 }
 
 __host__ __device__ inline uint64_t trace_bytes(uint32_t qlen, uint32_t tlen,
-                                                int R)
+                                                int R, int L)
 {
-    const uint32_t rows = 128u * (uint32_t)R;
+    const uint32_t rows = 2u * (uint32_t)R * (uint32_t)L;
     const uint32_t full = (qlen - 1) / rows;
-    const uint32_t nl = trace_last_lanes(qlen, R);
-    return ((uint64_t)full * trace_seg_steps(tlen + 127u) * 64u +
+    const uint32_t nl = trace_last_lanes(qlen, R, L);
+    return ((uint64_t)full * trace_seg_steps(tlen + 2u * L - 1u) * L +
             (uint64_t)trace_seg_steps(tlen + 2u * (nl - 1u) + 1u) * nl) *
            (uint64_t)R;
 }
 
 // Record index (in units of R bytes) of (pass p, step t >= 1, lane).
 __host__ __device__ inline uint64_t trace_rec(uint32_t qlen, uint32_t tlen,
-                                              int R, uint32_t p, uint32_t t,
-                                              uint32_t lane)
+                                              int R, int L, uint32_t p,
+                                              uint32_t t, uint32_t lane)
 {
-    const uint32_t rows = 128u * (uint32_t)R;
+    const uint32_t rows = 2u * (uint32_t)R * (uint32_t)L;
     const uint32_t full = (qlen - 1) / rows;
-    const uint32_t nact = p < full ? 64u : trace_last_lanes(qlen, R);
+    const uint32_t nact = p < full ? (uint32_t)L : trace_last_lanes(qlen, R, L);
     const uint32_t seg = (t - 1u) / kTraceSeg, off = (t - 1u) % kTraceSeg;
-    return (uint64_t)p * trace_seg_steps(tlen + 127u) * 64u +
+    return (uint64_t)p * trace_seg_steps(tlen + 2u * L - 1u) * L +
            (uint64_t)seg * nact * kTraceSeg + lane * kTraceSeg + off;
-}
-
-// DP kernel class of a task by its row count:
-//   0: qlen <= 128, 16 lanes x 4 rows (four tasks per wave, one pass);
-//   1: qlen <= 256, 16 lanes x 8 rows (four tasks per wave, one pass);
-//   2: qlen <= 512, 64 lanes x 4 rows (one pass);
-//   3: longer,      64 lanes x 8 rows (1024 rows per pass).
-__host__ __device__ inline int dp_class(uint32_t qlen)
-{
-    return qlen <= 128 ? 0 : (qlen <= 256 ? 1 : (qlen <= 512 ? 2 : 3));
-}
-
-// Rows per lane of the class.  The 16-lane classes are single-pass, so their
-// trace records fall exactly where the 64-lane layout with the same R puts
-// them (no full pass; ceil(qlen / 2R) active lanes).
-__host__ __device__ inline int rows_per_lane(uint32_t qlen)
-{
-    const int c = dp_class(qlen);
-    return c == 0 || c == 2 ? 4 : 8;
 }
 
 // X-drop extension task: one seed of one pair.
@@ -204,8 +213,8 @@ static_assert(sizeof(XdOut) == 40, "XdOut layout");
 // Launchers (k_dp.hip, k_xdrop.hip, k_misc.hip).
 hipError_t launch_dp(int cls, bool local, bool trace, bool i16,
                      const DpArgs &a, int grid, hipStream_t s);
-// Traceback of the tasks of one DP class (rows_per_lane: that class's R).
-hipError_t launch_traceback(const TbArgs &a, int rows_per_lane, int grid,
+// Traceback of the tasks of one DP class (the trace layout of its shape).
+hipError_t launch_traceback(const TbArgs &a, int cls, int grid,
                             hipStream_t s);
 // X-drop with one 16-lane row of a wave per work item (k_xdrop_coop.hip);
 // items whose band outgrows 16 columns are appended to ovf_list.

@@ -25,12 +25,13 @@ namespace {
 // Limits of one call.
 constexpr uint64_t kMaxPairs = 262144;     // pairs per call
 constexpr uint32_t kMaxLen = 4096;         // longest loaded sequence
-constexpr int kMaxWg = 1024;               // persistent DP workgroups
+constexpr int kMaxSlots = 1024;            // boundary rows of kMaxLen
+constexpr int kSlots = 4;                  // task slots per DP workgroup
 
 constexpr size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // Regions of the scratch array (byte offsets), sized for kMaxPairs pairs.
-constexpr size_t kCtrWords = 1024;         // >= the counter block's words
+constexpr size_t kCtrWords = 2048;         // >= the counter block's words
 constexpr size_t kOffTasks = 0;
 constexpr size_t kOffOut = align256(kOffTasks + kMaxPairs * sizeof(DpTask));
 constexpr size_t kOffFlags = align256(kOffOut + kMaxPairs * sizeof(DpOut));
@@ -41,7 +42,7 @@ constexpr size_t kOffCtr =
     align256(kOffSat + (size_t)kClasses * kMaxPairs * sizeof(uint32_t));
 constexpr size_t kOffBnd = align256(kOffCtr + kCtrWords * sizeof(uint32_t));
 constexpr size_t kBndBytes =
-    (size_t)kMaxWg * 2u * (kMaxLen + 2u) * sizeof(int32_t);
+    (size_t)kMaxSlots * 2u * (kMaxLen + 2u) * sizeof(int32_t);
 constexpr size_t kScratchBytes = align256(kOffBnd + kBndBytes);
 static_assert(kScratchBytes <= (size_t)64 << 20, "scratch stays modest");
 
@@ -76,7 +77,13 @@ public:
         quiet_ = c->quiet;
         wg_cap_ = c->wg_cap;
         c->quiet = true;
-        const int per_cu = std::max(1, kMaxWg / std::max(1, c->ncu));
+        // as many persistent workgroups per CU as the boundary-row region
+        // holds for the loaded sets' longest column sequence
+        const size_t row = 2u * ((size_t)c->max_clen + 2u) * sizeof(int32_t);
+        const size_t slots = kBndBytes / row;
+        const int per_cu = (int)std::max<size_t>(
+            1, std::min<size_t>(32, slots / ((size_t)kSlots *
+                                             std::max(1, c->ncu))));
         c->wg_cap = wg_cap_ > 0 ? std::min(wg_cap_, per_cu) : per_cu;
     }
     ~ScratchView()

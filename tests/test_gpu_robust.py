@@ -222,3 +222,32 @@ def test_device_entry_points_align_bad_pairs_as_empty():
         al.device_free(b)
     for b in (d_s, d_p2):
         al2.device_free(b)
+
+
+def test_graph_per_batch_of_a_resident_pair_array():
+    """One graph per batch offset (the --bsz loop over resident pairs):
+    each is captured once and replayed; all replays equal eager runs."""
+    import ctypes as C
+    w = small_config1(n_seq=120, n_pairs=3000, seed=66)
+    al = Aligner(config_for_cli(-11, -2, 6, 0, 2, "as_called"))
+    al.load_seqs(w.codes, w.offsets)
+    n, b = len(w.pairs), 1000
+    d_pairs = al.device_alloc(w.pairs.nbytes)
+    d_g = al.device_alloc(n * SCORE_DTYPE.itemsize)
+    d_e = al.device_alloc(n * SCORE_DTYPE.itemsize)
+    al.h2d(d_pairs, w.pairs)
+    want = _scores_eager(al, d_pairs, n, d_e)
+    caps0 = al.stats()["graph_captures"]
+    for rep in range(3):
+        print("batched graph replay", rep, flush=True)
+        for off in range(0, n, b):
+            dp = C.c_void_p(d_pairs.value + off * w.pairs.itemsize)
+            do = C.c_void_p(d_g.value + off * SCORE_DTYPE.itemsize)
+            al.sw_scores_graph(dp, b, do)
+        got = np.zeros(n, SCORE_DTYPE)
+        al.synchronize()
+        al.d2h(got, d_g)
+        assert np.array_equal(got, want), rep
+    assert al.stats()["graph_captures"] == caps0 + 3
+    for x in (d_pairs, d_g, d_e):
+        al.device_free(x)
