@@ -183,8 +183,14 @@ __device__ inline int bin_key(const DpTask &t, int local, int maxs, int open,
         local ? true
               : global_i16_ok(t.qlen, t.tlen, t.flags, maxs, open, ext);
     const int bin = 2 * rc + (i16 ? 0 : 1);
-    const float lg = __log2f((float)task_steps(t.qlen, t.tlen) * (1.0f / 16.f));
-    const int b = (int)(8.0f * (lg > 0.f ? lg : 0.f));
+    // 8 * log2(steps / 16) from the leading bit and the next three
+    const uint32_t x = task_steps(t.qlen, t.tlen) >> 4;
+    int b = 0;
+    if (x) {
+        const int e = 31 - __builtin_clz(x);
+        const int f = (int)((e >= 3 ? x >> (e - 3) : x << (3 - e)) & 7u);
+        b = 8 * e + f;
+    }
     const int bucket = kBuckets - 1 - (b < kBuckets - 1 ? b : kBuckets - 1);
     return bin * kBuckets + bucket;
 }
