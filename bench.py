@@ -294,9 +294,20 @@ def measure_sw_config(args, which):
             else:
                 al.sw_scores_dev(dp, m, do)
 
-    # eager pass with kernel timers: the roofline of the dominant kernel
+    # eager passes with kernel timers: the default launch shapes, then the
+    # autotuned ones (whose pass gives the roofline of the dominant kernel)
     run(False)
     al.synchronize()
+    t0 = time.perf_counter()
+    run(False)
+    al.synchronize()
+    default_ms = (time.perf_counter() - t0) * 1e3
+    if not args.no_autotune:
+        from pastis_amd import autotune
+        autotune.tune(al, (d_pairs, min(batch, n)), xdrop=False, force=True,
+                      log=log)
+        run(False)
+        al.synchronize()
     al.reset_stats()
     t0 = time.perf_counter()
     run(False)
@@ -323,6 +334,8 @@ def measure_sw_config(args, which):
         "value": round(cells * steps / el / 1e9, 3), "unit": "GCUPS",
         "pairs_per_s": round(n * steps / el, 1), "cells_per_step": cells,
         "eager_ms_per_step": round(eager_ms, 3),
+        "eager_ms_per_step_default_tuning": round(default_ms, 3),
+        "tuning": al.tuning(),
         "int32_rerun_pairs": reruns,
         "int32_rerun_rate": round(reruns / n, 8),
         "dtype": "int16 (packed; int32 re-run on saturation)",
@@ -532,6 +545,8 @@ def main():
                          "(HBM counter calibration) and exit")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes")
+    ap.add_argument("--no-operator", action="store_true",
+                    help="skip timing HipAligner::aln_batch on host tuples")
     ap.add_argument("--no-filter", action="store_true",
                     help="skip the learned-filter vs plain comparison")
     ap.add_argument("--configs", default="c1,c3,c5",
@@ -582,16 +597,6 @@ def main():
     out_dtype = RESULT_DTYPE if args.workload == "xdrop" else SCORE_DTYPE
     d_out = al.device_alloc(n * out_dtype.itemsize)
     al.h2d(d_pairs, pairs)
-    tune_table = []
-    if not args.no_autotune:
-        from pastis_amd import autotune
-        t = time.time()
-        choice, tune_table = autotune.tune(al, pairs,
-                                           xdrop=args.workload == "xdrop",
-                                           log=log)
-        log("autotune picked %s in %.1f s" % (choice, time.time() - t))
-    tuning = al.tuning()
-
     def step():
         if args.workload == "xdrop":
             al.align_dev(d_pairs, n, d_out)
@@ -599,6 +604,21 @@ def main():
             al.sw_scores_graph(d_pairs, n, d_out)
         else:
             al.sw_scores_dev(d_pairs, n, d_out)
+
+    # one step with the default launch shapes (after a warm-up step), for
+    # the autotuner's gain
+    step()
+    al.synchronize()
+    t0 = time.perf_counter()
+    step()
+    al.synchronize()
+    default_ms = (time.perf_counter() - t0) * 1e3
+    tune_s = None
+    if not args.no_autotune:
+        from pastis_amd import autotune
+        _, tune_s = autotune.tune(al, (d_pairs, n),
+                                  xdrop=args.workload == "xdrop", log=log)
+    tuning = al.tuning()
 
     for i in range(args.warmup):
         step()
@@ -637,6 +657,8 @@ def main():
                    "xdrop": 49 if args.workload == "xdrop" else None,
                    "seed_count": 2, "k": 6, "matrix": "BLOSUM62",
                    "tuning": tuning},
+        "ms_per_step_default_tuning": round(default_ms, 3),
+        "autotune_seconds": None if tune_s is None else round(tune_s, 2),
         "pairs_per_s": round(n * args.steps / elapsed, 1),
         "cells_per_step": int(cells / args.steps),
         "xdrop_cells_per_step": int(st["xdrop_cells"] / args.steps),
@@ -695,6 +717,26 @@ def main():
     al.device_free(d_pairs)
     al.device_free(d_out)
     al.close()
+    if args.workload == "xdrop" and not args.no_operator:
+        # the PWAlign operator on host tuples (pack, page-locked staging,
+        # alignment, score_aln write-back), same pairs and launch shapes
+        from pastis_amd import pwalign
+        t = time.time()
+        op_ms, op_passed = pwalign.bench_aln_batch(
+            1, w.codes, w.offsets, pairs, gap_model=args.gap_model,
+            autotune=not args.no_autotune, warmup=1,
+            steps=max(1, min(args.steps, 5)))
+        line["operator_path"] = {
+            "entry": "HipAligner::aln_batch (CommonKmerLoc tuples in host "
+                     "memory)",
+            "ms_per_call": round(op_ms, 3),
+            "pairs_per_s": round(n / (op_ms * 1e-3), 1),
+            "passed": int(op_passed),
+            "passed_matches_device_path": int(op_passed) ==
+            line["passing_pairs"],
+            "vs_device_resident_ms_per_step": round(op_ms / ms_per_step, 3)}
+        log("operator path: %.1f ms per aln_batch call (%.1f s)" %
+            (op_ms, time.time() - t))
     extra = [c.strip() for c in args.configs.split(",")
              if c.strip() and c.strip() != "none"]
     if extra:

@@ -381,14 +381,21 @@ int pa_filter_select_dev(struct pa_ctx *ctx, const pa_pair *d_pairs,
 /* ---- device memory helpers (for the _dev entry points) --------------- */
 int pa_device_alloc(struct pa_ctx *ctx, size_t bytes, void **ptr);
 int pa_device_free(struct pa_ctx *ctx, void *ptr);
+/* Page-locked host memory: copies between it and the device run at full
+ * link speed and without a staging copy (HipAligner stages its batches
+ * through it). */
+int pa_host_alloc(struct pa_ctx *ctx, size_t bytes, void **ptr);
+int pa_host_free(struct pa_ctx *ctx, void *ptr);
 int pa_memcpy_h2d(struct pa_ctx *ctx, void *dst, const void *src, size_t bytes);
 int pa_memcpy_d2h(struct pa_ctx *ctx, void *dst, const void *src, size_t bytes);
 int pa_synchronize(struct pa_ctx *ctx);
 
 /* Per kernel class: DP kernels are split by mode (sw = local score only,
- * swt = local traced, nwt = global traced), rows per lane (r2, r4, r8) and
- * arithmetic (i16 packed, i32); "rerun" classes are the int32 re-runs of
- * saturated tasks.  cells are DP cells of the tasks the class ran. */
+ * swt = local traced, nwt = global traced), row class (c0..c3, with the
+ * class's current kernel shape, e.g. l64r8 = 64 lanes x 8 rows per lane)
+ * and arithmetic (i16 packed, i32); "rerun" classes are the int32 re-runs
+ * of saturated tasks.  Example: "nwt.c3.l64r8.i16".  cells are DP cells of
+ * the tasks the class ran. */
 typedef struct pa_kernel_stat {
     char name[24];
     uint64_t launches;
@@ -398,8 +405,19 @@ typedef struct pa_kernel_stat {
 
 /* ---- tuning ---------------------------------------------------------
  * Launch shapes chosen per context (no counterpart in the reference; the
- * autotuner of pastis_amd/autotune.py searches them).  Results never depend
- * on them. */
+ * in-library autotuner pa_autotune below searches them).  Results never
+ * depend on them, only the speed does. */
+
+/* Kernel shapes of the DP kernels: L lanes per task (64 / L tasks per wave)
+ * and R rows per lane per half-strip, so one pass covers 2 * R * L rows;
+ * longer tasks take several passes. */
+enum pa_dp_shape {
+    PA_SHAPE_L16R4 = 0,     /* 128 rows per pass */
+    PA_SHAPE_L16R8 = 1,     /* 256 */
+    PA_SHAPE_L64R4 = 2,     /* 512 */
+    PA_SHAPE_L64R8 = 3      /* 1024 */
+};
+
 typedef struct pa_tuning {
     int32_t dp_wg_per_cu;   /* persistent DP workgroups (one wave) per CU,
                                1..32; 0 = as many as occupancy allows */
@@ -409,10 +427,42 @@ typedef struct pa_tuning {
                                kernel first (band up to 14 columns), the rest
                                to the one-lane-per-item kernels; 0: one lane
                                per item only */
+    int32_t dp_edge[3];     /* DP row classes: a task whose row span has
+                               qlen rows is in the first class c with
+                               qlen <= dp_edge[c], class 3 takes the rest;
+                               strictly increasing, 1..65535; default
+                               128, 256, 512 */
+    int32_t dp_shape[4];    /* pa_dp_shape of each row class; default
+                               L16R4, L16R8, L64R4, L64R8 */
+    int32_t tb_stage;       /* 1: the traceback copies each lane run of the
+                               trace it enters into LDS with wide loads;
+                               0: one byte load per step */
 } pa_tuning;
 
 int pa_get_tuning(struct pa_ctx *ctx, pa_tuning *out);
 int pa_set_tuning(struct pa_ctx *ctx, const pa_tuning *t);
+
+/* What pa_autotune times. */
+enum pa_tune_target {
+    PA_TUNE_SW_SCORES = 0,  /* pa_sw_scores_dev */
+    PA_TUNE_ALIGN = 1       /* pa_align_dev in the context's mode */
+};
+#define PA_TUNE_FORCE 1u    /* time again even when the cache has a choice */
+
+/* Autotuner (no counterpart in the reference): times the pairs at d_pairs
+ * (device memory; at most `sample` of them, taken evenly spaced; 0 = a
+ * default of 262144) under candidate tunings and installs the fastest on
+ * the context.  The DP row-class edges, each class's kernel shape, the
+ * persistent-grid size, the traceback staging and the X-drop kernel choice
+ * are each chosen by the device time of the stage they affect.  The choice
+ * is cached per device, target and alignment mode, so later contexts on the
+ * same device (and HipAligner objects) install it without timing; `flags`
+ * PA_TUNE_FORCE times again.  The caller's stream work is finished first;
+ * the context's sequences must be loaded; its statistics (pa_get_stats) are
+ * reset afterwards.  Sample buffers are allocated for the call.  *out
+ * (optional) receives the installed tuning. */
+int pa_autotune(struct pa_ctx *ctx, const pa_pair *d_pairs, uint64_t n,
+                int target, uint64_t sample, uint32_t flags, pa_tuning *out);
 
 /* ---- counters --------------------------------------------------------- */
 

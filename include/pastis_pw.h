@@ -11,6 +11,8 @@
 
 #include <stdint.h>
 
+#include "pastis_hip.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -44,6 +46,21 @@ int pw_run_block(int mode, int gap_open, int gap_ext, int klength, int xdrop,
                  const float *filter_params, const float *filter_mean,
                  const float *filter_istd, float filter_tau,
                  float *score_aln_out, uint64_t *stats);
+
+/* Times the operator path on host tuples: HipAligner::aln_batch
+ * (inc/align/PWAlign.hpp:45-63) over one batch of nnz tuples made once from
+ * pairs (r, c = block-local indices of a diagonal block of the nseq
+ * sequences, count, kscore, seeds), warmup untimed and steps timed calls.
+ * ms_per_call receives the mean wall time of a timed call: packing into
+ * pa_pair, page-locked staging copies, the GPU alignment and the write-back
+ * of score_aln.  passed (may be NULL) receives the pairs the last call
+ * wrote.  autotune = 0 turns the aligner's first-call autotuning off. */
+int pw_bench_aln_batch(int mode, int gap_open, int gap_ext, int klength,
+                       int xdrop, int seed_count, int gap_model, int device,
+                       const char *res, const uint64_t *offs, uint64_t nseq,
+                       const pa_pair *pairs, uint64_t nnz, double cov_thr,
+                       int ani_thr, int autotune, int warmup, int steps,
+                       double *ms_per_call, uint64_t *passed);
 
 #ifdef __cplusplus
 }

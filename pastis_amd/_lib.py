@@ -24,13 +24,14 @@ EXPORTS = [
     "pa_create", "pa_config_for_cli", "pa_destroy", "pa_strerror",
     "pa_last_error", "pa_load_seqs", "pa_align", "pa_align_dev",
     "pa_sw_scores", "pa_sw_scores_dev", "pa_xdrop_extents", "pa_kmer_prune",
-    "pa_device_alloc", "pa_device_free", "pa_memcpy_h2d", "pa_memcpy_d2h",
+    "pa_device_alloc", "pa_device_free", "pa_host_alloc", "pa_host_free",
+    "pa_memcpy_h2d", "pa_memcpy_d2h",
     "pa_synchronize", "pa_get_stats", "pa_reset_stats",
     "pa_get_kernel_stats", "pa_filter_features", "pa_filter_features_dev",
     "pa_filter_set_model", "pa_filter_get_model", "pa_filter_grad",
     "pa_filter_train", "pa_filter_logits", "pa_filter_score_dev",
     "pa_filter_select_dev", "pa_set_thresholds", "pa_get_tuning",
-    "pa_set_tuning", "pa_sw_scores_graph", "pa_kmer_candidates",
+    "pa_set_tuning", "pa_autotune", "pa_sw_scores_graph", "pa_kmer_candidates",
     "pa_fasta_parse", "pa_fasta_info", "pa_fasta_residues",
     "pa_fasta_offsets", "pa_fasta_records", "pa_fasta_free", "pa_load_fasta",
     "pa_triples_dev", "pa_write_mm", "pa_sw_scores_noalloc",
@@ -85,12 +86,36 @@ class PaKernelStat(C.Structure):
                 ("cells", C.c_uint64), ("ms", C.c_double)]
 
 
+DP_SHAPES = {"l16r4": 0, "l16r8": 1, "l64r4": 2, "l64r8": 3}
+TUNE_TARGETS = {"sw_scores": 0, "align": 1}
+
+
 class PaTuning(C.Structure):
     _fields_ = [("dp_wg_per_cu", C.c_int32), ("xd_wg_per_cu", C.c_int32),
-                ("xd_ring", C.c_int32), ("xd_coop", C.c_int32)]
+                ("xd_ring", C.c_int32), ("xd_coop", C.c_int32),
+                ("dp_edge", C.c_int32 * 3), ("dp_shape", C.c_int32 * 4),
+                ("tb_stage", C.c_int32)]
 
     def as_dict(self):
-        return {n: getattr(self, n) for n, _ in self._fields_}
+        """Scalars as ints, dp_edge / dp_shape as tuples."""
+        out = {}
+        for n, _ in self._fields_:
+            v = getattr(self, n)
+            out[n] = tuple(v) if n in ("dp_edge", "dp_shape") else v
+        return out
+
+    def update(self, **kw):
+        for k, v in kw.items():
+            if k in ("dp_edge", "dp_shape"):
+                arr = getattr(self, k)
+                if len(v) != len(arr):
+                    raise ValueError("%s takes %d values" % (k, len(arr)))
+                for i, x in enumerate(v):
+                    arr[i] = DP_SHAPES[x] if isinstance(x, str) else int(x)
+            elif k in dict(self._fields_):
+                setattr(self, k, int(v))
+            else:
+                raise KeyError("no tuning field %r" % k)
 
 
 class PaKmerConfig(C.Structure):
@@ -158,6 +183,8 @@ def lib():
     L.pa_set_thresholds.argtypes = [vp, C.c_double, i32]
     L.pa_get_tuning.argtypes = [vp, C.POINTER(PaTuning)]
     L.pa_set_tuning.argtypes = [vp, C.POINTER(PaTuning)]
+    L.pa_autotune.argtypes = [vp, vp, C.c_uint64, C.c_int, C.c_uint64,
+                              C.c_uint32, C.POINTER(PaTuning)]
     fp = C.POINTER(C.c_float)
     L.pa_filter_features.argtypes = [vp, vp, u64, vp]
     L.pa_filter_features_dev.argtypes = [vp, vp, u64, vp, vp]

@@ -155,10 +155,18 @@ class Aligner:
         t = _lib.PaTuning()
         check(self._ctx, lib().pa_get_tuning(self._ctx, C.byref(t)),
               "pa_get_tuning")
-        for k, v in kw.items():
-            setattr(t, k, int(v))
+        t.update(**kw)
         check(self._ctx, lib().pa_set_tuning(self._ctx, C.byref(t)),
               "pa_set_tuning")
+
+    def autotune(self, d_pairs, n, target="align", sample=0, force=False):
+        """pa_autotune on n device-resident pairs: installs the fastest
+        tuning for this context and returns it (a tuning() dict)."""
+        t = _lib.PaTuning()
+        check(self._ctx, lib().pa_autotune(
+            self._ctx, d_pairs, n, _lib.TUNE_TARGETS[target], sample,
+            1 if force else 0, C.byref(t)), "pa_autotune")
+        return t.as_dict()
 
     # ---- k-mer candidate generation ---------------------------------------
     def kmer_candidates(self, k=6, stride=1, alphabet="pdefault",

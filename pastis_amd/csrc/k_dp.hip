@@ -766,28 +766,27 @@ namespace pa {
 namespace {
 
 
-// R is fixed per launch (one launch per DP class), so the layout arithmetic
-// divides by constants.  Global traces mark a diagonal step between equal
-// residues with tag 3 (restart does not occur there), so LOCAL = false needs
-// no residue loads.
+// R and L are fixed per launch (one launch per DP class, whose kernel
+// shape wrote the trace), so the layout arithmetic divides by constants.
+// Global traces mark a diagonal step between equal residues with tag 3
+// (restart does not occur there), so LOCAL = false needs no residue loads.
 //
-// The walk reads the trace one lane run at a time: the kTraceSeg records of
-// one lane in one segment (8R contiguous bytes, 8R-aligned) are loaded with
-// wide loads into the thread's slot in LDS when the walk enters them, and
-// the codes are read from there.  Diagonal and horizontal moves step back
-// one record inside the run, vertical moves stay in the record, so a run
-// serves several steps; reading single bytes from global memory instead
-// lets the lines of thousands of concurrent walks evict each other before
-// their next byte is read.
+// STAGE: the walk reads the trace one lane run at a time: the kTraceSeg
+// records of one lane in one segment (8R contiguous bytes, 8R-aligned) are
+// loaded with wide loads into the thread's slot in LDS when the walk enters
+// them, and the codes are read from there.  Diagonal and horizontal moves
+// step back one record inside the run, vertical moves stay in the record,
+// so a run serves several steps.  Without STAGE each step loads its byte
+// from global memory.
 constexpr int kTbThreads = 256;
 
-template <int R, int L, bool LOCAL>
+template <int R, int L, bool LOCAL, bool STAGE>
 __global__ __launch_bounds__(kTbThreads) void k_traceback(TbArgs a)
 {
     constexpr int RUNW = 2 * R;   // 32-bit words of one lane run (8R bytes)
     // one run per thread; the odd stride spreads threads over the banks
-    __shared__ uint32_t stage[kTbThreads * (RUNW + 1)];
-    uint32_t *run = stage + threadIdx.x * (RUNW + 1);
+    __shared__ uint32_t stage[STAGE ? kTbThreads * (RUNW + 1) : 1];
+    uint32_t *run = stage + (STAGE ? threadIdx.x * (RUNW + 1) : 0);
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= *a.count)
         return;
@@ -865,23 +864,28 @@ __global__ __launch_bounds__(kTbThreads) void k_traceback(TbArgs a)
             const uint32_t t1 = (uint32_t)(j + 2 * l + h) - 1u;
             const uint32_t run_off =
                 pbase + (t1 / kTraceSeg) * seg_bytes + (uint32_t)l * RUN;
-            if (run_off != cur_run) {
-                cur_run = run_off;
-                // generic pointer: vector-struct loads from the global
-                // address space do not compile
-                const uint4 *src = (const uint4 *)(trg + run_off);
-#pragma unroll
-                for (int q = 0; q < RUNW / 4; ++q) {
-                    const uint4 x = src[q];
-                    run[4 * q + 0] = x.x;
-                    run[4 * q + 1] = x.y;
-                    run[4 * q + 2] = x.z;
-                    run[4 * q + 3] = x.w;
-                }
-            }
             const uint32_t bi = (t1 % kTraceSeg) * R + 4u * (r >> 2) +
                                 2u * (r & 1) + (uint32_t)h;
-            const uint32_t byte = (run[bi >> 2] >> (8u * (bi & 3u))) & 0xffu;
+            uint32_t byte;
+            if (STAGE) {
+                if (run_off != cur_run) {
+                    cur_run = run_off;
+                    // generic pointer: vector-struct loads from the global
+                    // address space do not compile
+                    const uint4 *src = (const uint4 *)(trg + run_off);
+#pragma unroll
+                    for (int q = 0; q < RUNW / 4; ++q) {
+                        const uint4 x = src[q];
+                        run[4 * q + 0] = x.x;
+                        run[4 * q + 1] = x.y;
+                        run[4 * q + 2] = x.z;
+                        run[4 * q + 3] = x.w;
+                    }
+                }
+                byte = (run[bi >> 2] >> (8u * (bi & 3u))) & 0xffu;
+            } else {
+                byte = glb(trg)[run_off + bi];
+            }
             const int c = (int)((r & 2) ? (byte >> 4) : (byte & 15u));
             if (state == 0) {
                 const int src = c & 3;
@@ -973,60 +977,63 @@ hipError_t launch_c(bool local, bool trace, bool i16, const DpArgs &a,
                : launch_one<R, L, false, true, false>(a, grid, s);
 }
 
-// kernel shape of each DP class (class_rows / class_lanes)
-template <int CLS> struct Shape {
-    static constexpr int R = class_rows(CLS), L = class_lanes(CLS);
-};
-
-template <int CLS>
-hipError_t launch_traceback_c(const TbArgs &a, int grid, hipStream_t s)
+template <int R, int L>
+hipError_t launch_traceback_c(const TbArgs &a, bool stage, int grid,
+                              hipStream_t s)
 {
-    constexpr int R = Shape<CLS>::R, L = Shape<CLS>::L;
-    if (a.local)
-        hipLaunchKernelGGL((k_traceback<R, L, true>), dim3(grid),
-                           dim3(kTbThreads), 0, s, a);
-    else
-        hipLaunchKernelGGL((k_traceback<R, L, false>), dim3(grid),
-                           dim3(kTbThreads), 0, s, a);
+    const dim3 g(grid), b(kTbThreads);
+    if (a.local) {
+        if (stage)
+            hipLaunchKernelGGL((k_traceback<R, L, true, true>), g, b, 0, s, a);
+        else
+            hipLaunchKernelGGL((k_traceback<R, L, true, false>), g, b, 0, s, a);
+    } else {
+        if (stage)
+            hipLaunchKernelGGL((k_traceback<R, L, false, true>), g, b, 0, s, a);
+        else
+            hipLaunchKernelGGL((k_traceback<R, L, false, false>), g, b, 0, s,
+                               a);
+    }
     return hipGetLastError();
 }
 
 }  // namespace
 
-hipError_t launch_dp(int cls, bool local, bool trace, bool i16,
+hipError_t launch_dp(int shape, bool local, bool trace, bool i16,
                      const DpArgs &a, int grid, hipStream_t s)
 {
-    switch (cls) {
-    case 0: return launch_c<Shape<0>::R, Shape<0>::L>(local, trace, i16, a, grid, s);
-    case 1: return launch_c<Shape<1>::R, Shape<1>::L>(local, trace, i16, a, grid, s);
-    case 2: return launch_c<Shape<2>::R, Shape<2>::L>(local, trace, i16, a, grid, s);
-    case 3: return launch_c<Shape<3>::R, Shape<3>::L>(local, trace, i16, a, grid, s);
+    switch (shape) {
+    case kShapeL16R4: return launch_c<4, 16>(local, trace, i16, a, grid, s);
+    case kShapeL16R8: return launch_c<8, 16>(local, trace, i16, a, grid, s);
+    case kShapeL64R4: return launch_c<4, 64>(local, trace, i16, a, grid, s);
+    case kShapeL64R8: return launch_c<8, 64>(local, trace, i16, a, grid, s);
     default: return hipErrorInvalidValue;
     }
 }
 
-int dp_occupancy(int cls, bool local, bool trace, bool i16)
+int dp_occupancy(int shape, bool local, bool trace, bool i16)
 {
-    static int cache[4][8] = {};
+    static int cache[kShapes][8] = {};
     const int vi = (local ? 4 : 0) + (trace ? 2 : 0) + (i16 ? 1 : 0);
-    if (cls < 0 || cls > 3)
+    if (shape < 0 || shape >= kShapes)
         return 1;
-    if (!cache[cls][vi])
-        cache[cls][vi] =
-            cls == 0   ? occupancy_c<Shape<0>::R, Shape<0>::L>(local, trace, i16)
-            : cls == 1 ? occupancy_c<Shape<1>::R, Shape<1>::L>(local, trace, i16)
-            : cls == 2 ? occupancy_c<Shape<2>::R, Shape<2>::L>(local, trace, i16)
-                       : occupancy_c<Shape<3>::R, Shape<3>::L>(local, trace, i16);
-    return cache[cls][vi];
+    if (!cache[shape][vi])
+        cache[shape][vi] =
+            shape == kShapeL16R4   ? occupancy_c<4, 16>(local, trace, i16)
+            : shape == kShapeL16R8 ? occupancy_c<8, 16>(local, trace, i16)
+            : shape == kShapeL64R4 ? occupancy_c<4, 64>(local, trace, i16)
+                                   : occupancy_c<8, 64>(local, trace, i16);
+    return cache[shape][vi];
 }
 
-hipError_t launch_traceback(const TbArgs &a, int cls, int grid, hipStream_t s)
+hipError_t launch_traceback(const TbArgs &a, int shape, bool stage, int grid,
+                            hipStream_t s)
 {
-    switch (cls) {
-    case 0: return launch_traceback_c<0>(a, grid, s);
-    case 1: return launch_traceback_c<1>(a, grid, s);
-    case 2: return launch_traceback_c<2>(a, grid, s);
-    case 3: return launch_traceback_c<3>(a, grid, s);
+    switch (shape) {
+    case kShapeL16R4: return launch_traceback_c<4, 16>(a, stage, grid, s);
+    case kShapeL16R8: return launch_traceback_c<8, 16>(a, stage, grid, s);
+    case kShapeL64R4: return launch_traceback_c<4, 64>(a, stage, grid, s);
+    case kShapeL64R8: return launch_traceback_c<8, 64>(a, stage, grid, s);
     default: return hipErrorInvalidValue;
     }
 }

@@ -5,7 +5,8 @@
 
 namespace pa {
 
-// Bins: 2 * DP class (dp_class) + (1 for the int32 kernels).  Inside a bin
+// Bins: 2 * DP class (dp_class under the context's DpShapes) + (1 for the
+// int32 kernels).  Inside a bin
 // the tasks are ordered by kBuckets log-spaced buckets of their step count,
 // longest first: the tasks a wave runs together are alike, and the
 // persistent grid ends on short tasks.
@@ -35,13 +36,14 @@ hipError_t build_nw_tasks(const XdTask *xt, uint64_t nt, const XdOut *xo,
 // Counting sort of tasks [a, b) into per-bin lists: hist and fill are
 // kBins * kBuckets counters that start at zero, counts receives the kBins
 // list lengths.
-hipError_t bin_tasks(const DpTask *tasks, uint32_t a, uint32_t b, int local,
-                     int maxs, int open, int ext, uint32_t *lists,
+hipError_t bin_tasks(const DpTask *tasks, uint32_t a, uint32_t b,
+                     const DpShapes &sh, int local, int maxs, int open,
+                     int ext, uint32_t *lists,
                      uint32_t cap, uint32_t *counts, uint32_t *hist,
                      uint32_t *fill, unsigned long long *bin_cells,
                      hipStream_t s);
-hipError_t trace_sizes(const DpTask *tasks, uint64_t n, uint64_t *sizes,
-                       hipStream_t s);
+hipError_t trace_sizes(const DpTask *tasks, uint64_t n, const DpShapes &sh,
+                       uint64_t *sizes, hipStream_t s);
 hipError_t empty_out(const DpTask *tasks, uint32_t a, uint32_t b, int local,
                      int open, int ext, DpOut *out, hipStream_t s);
 hipError_t mark_flags(const uint32_t *list, const uint32_t *count,

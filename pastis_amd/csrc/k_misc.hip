@@ -162,9 +162,10 @@ __device__ inline bool global_i16_ok(uint32_t q, uint32_t t, uint32_t flags,
 }
 
 // Wavefront steps of a task in its class's kernel, all passes.
-__device__ inline uint32_t task_steps(uint32_t qlen, uint32_t tlen)
+__device__ inline uint32_t task_steps(uint32_t qlen, uint32_t tlen,
+                                      const DpShapes &sh)
 {
-    const int R = rows_per_lane(qlen), L = lanes_per_task(qlen);
+    const int R = rows_per_lane(qlen, sh), L = lanes_per_task(qlen, sh);
     const uint32_t rows = 2u * R * L;
     const uint32_t npass = (qlen + rows - 1) / rows;
     const uint32_t nl = trace_last_lanes(qlen, R, L);
@@ -173,18 +174,18 @@ __device__ inline uint32_t task_steps(uint32_t qlen, uint32_t tlen)
 
 // Bin and bucket of a task (-1: empty task).  Buckets are 1/8 of an octave
 // of the step count wide (16 .. 2^23 steps), the longest in bucket 0.
-__device__ inline int bin_key(const DpTask &t, int local, int maxs, int open,
-                              int ext)
+__device__ inline int bin_key(const DpTask &t, const DpShapes &sh, int local,
+                              int maxs, int open, int ext)
 {
     if (t.qlen == 0 || t.tlen == 0)
         return -1;
-    const int rc = dp_class(t.qlen);
+    const int rc = dp_class(t.qlen, sh);
     const bool i16 =
         local ? true
               : global_i16_ok(t.qlen, t.tlen, t.flags, maxs, open, ext);
     const int bin = 2 * rc + (i16 ? 0 : 1);
     // 8 * log2(steps / 16) from the leading bit and the next three
-    const uint32_t x = task_steps(t.qlen, t.tlen) >> 4;
+    const uint32_t x = task_steps(t.qlen, t.tlen, sh) >> 4;
     int b = 0;
     if (x) {
         const int e = 31 - __builtin_clz(x);
@@ -198,15 +199,15 @@ __device__ inline int bin_key(const DpTask &t, int local, int maxs, int open,
 // Pass 1: histogram of the (bin, bucket) keys: LDS counters per block, then
 // one global atomic per key that occurs in the block.
 __global__ __launch_bounds__(256) void k_bin_hist(
-    const DpTask *__restrict__ tasks, uint32_t a, uint32_t b, int local,
-    int maxs, int open, int ext, uint32_t *hist)
+    const DpTask *__restrict__ tasks, uint32_t a, uint32_t b, DpShapes sh,
+    int local, int maxs, int open, int ext, uint32_t *hist)
 {
     __shared__ uint32_t h[kBins * kBuckets];
     for (int k = threadIdx.x; k < kBins * kBuckets; k += blockDim.x)
         h[k] = 0;
     __syncthreads();
     const uint32_t i = a + blockIdx.x * blockDim.x + threadIdx.x;
-    const int key = i < b ? bin_key(tasks[i], local, maxs, open, ext) : -1;
+    const int key = i < b ? bin_key(tasks[i], sh, local, maxs, open, ext) : -1;
     if (key >= 0)
         atomicAdd(&h[key], 1u);
     __syncthreads();
@@ -235,8 +236,9 @@ __global__ void k_bin_scan(const uint32_t *hist, uint32_t *fill,
 // block counts its keys in LDS (the LDS atomic's old value is the task's
 // rank), reserves one range per key with a global atomic, and scatters.
 __global__ __launch_bounds__(256) void k_bin(
-    const DpTask *__restrict__ tasks, uint32_t a, uint32_t b, int local,
-    int maxs, int open, int ext, uint32_t *__restrict__ lists, uint32_t cap,
+    const DpTask *__restrict__ tasks, uint32_t a, uint32_t b, DpShapes sh,
+    int local, int maxs, int open, int ext, uint32_t *__restrict__ lists,
+    uint32_t cap,
     uint32_t *fill, unsigned long long *bin_cells)
 {
     __shared__ uint32_t h[kBins * kBuckets], base[kBins * kBuckets];
@@ -251,7 +253,7 @@ __global__ __launch_bounds__(256) void k_bin(
     uint64_t cells = 0;
     if (i < b) {
         const DpTask t = tasks[i];
-        key = bin_key(t, local, maxs, open, ext);
+        key = bin_key(t, sh, local, maxs, open, ext);
         cells = (uint64_t)t.qlen * t.tlen;
     }
     uint32_t rank = 0;
@@ -270,14 +272,14 @@ __global__ __launch_bounds__(256) void k_bin(
 }
 
 __global__ void k_trace_sizes(const DpTask *__restrict__ tasks, uint64_t n,
-                              uint64_t *__restrict__ sizes)
+                              DpShapes sh, uint64_t *__restrict__ sizes)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
         const DpTask t = tasks[i];
         sizes[i] = (t.qlen && t.tlen)
-                       ? trace_bytes(t.qlen, t.tlen, rows_per_lane(t.qlen),
-                                     lanes_per_task(t.qlen))
+                       ? trace_bytes(t.qlen, t.tlen, rows_per_lane(t.qlen, sh),
+                                     lanes_per_task(t.qlen, sh))
                        : 0;
     }
 }
@@ -501,8 +503,9 @@ hipError_t build_nw_tasks(const XdTask *xt, uint64_t nt, const XdOut *xo,
     return hipGetLastError();
 }
 
-hipError_t bin_tasks(const DpTask *tasks, uint32_t a, uint32_t b, int local,
-                     int maxs, int open, int ext, uint32_t *lists,
+hipError_t bin_tasks(const DpTask *tasks, uint32_t a, uint32_t b,
+                     const DpShapes &sh, int local, int maxs, int open,
+                     int ext, uint32_t *lists,
                      uint32_t cap, uint32_t *counts, uint32_t *hist,
                      uint32_t *fill, unsigned long long *bin_cells,
                      hipStream_t s)
@@ -510,22 +513,22 @@ hipError_t bin_tasks(const DpTask *tasks, uint32_t a, uint32_t b, int local,
     if (b <= a)
         return hipSuccess;
     hipLaunchKernelGGL(k_bin_hist, dim3(blocks(b - a, 256)), dim3(256), 0, s,
-                       tasks, a, b, local, maxs, open, ext, hist);
+                       tasks, a, b, sh, local, maxs, open, ext, hist);
     hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(64), 0, s, hist, fill,
                        counts);
     hipLaunchKernelGGL(k_bin, dim3(blocks(b - a, 256)), dim3(256), 0, s, tasks,
-                       a, b, local, maxs, open, ext, lists, cap, fill,
+                       a, b, sh, local, maxs, open, ext, lists, cap, fill,
                        bin_cells);
     return hipGetLastError();
 }
 
-hipError_t trace_sizes(const DpTask *tasks, uint64_t n, uint64_t *sizes,
-                       hipStream_t s)
+hipError_t trace_sizes(const DpTask *tasks, uint64_t n, const DpShapes &sh,
+                       uint64_t *sizes, hipStream_t s)
 {
     if (!n)
         return hipSuccess;
     hipLaunchKernelGGL(k_trace_sizes, dim3(blocks(n, 256)), dim3(256), 0, s,
-                       tasks, n, sizes);
+                       tasks, n, sh, sizes);
     return hipGetLastError();
 }
 

@@ -186,7 +186,7 @@ int ensure_common(pa_ctx *c, uint64_t nt)
 int dp_grid(pa_ctx *c, uint64_t ntasks, int cls, bool local, bool trace,
             bool i16)
 {
-    int per_cu = dp_occupancy(cls, local, trace, i16);
+    int per_cu = dp_occupancy(c->shapes.shape[cls], local, trace, i16);
     if (c->tune.dp_wg_per_cu > 0)
         per_cu = std::min(per_cu, (int)c->tune.dp_wg_per_cu);
     if (c->wg_cap > 0)
@@ -232,7 +232,8 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
             return rc;
         if ((rc = grow(c, c->toffs, nt * sizeof(uint64_t) + 8)))
             return rc;
-        HIPCHK(c, trace_sizes(P<DpTask>(c->tasks), nt, P<uint64_t>(c->sizes), s));
+        HIPCHK(c, trace_sizes(P<DpTask>(c->tasks), nt, c->shapes,
+                              P<uint64_t>(c->sizes), s));
         size_t tmp = 0;
         HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(
                       nullptr, tmp, P<uint64_t>(c->sizes),
@@ -296,7 +297,8 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
             base = c->h_toffs[a];
         HIPCHK(c, fill_u32(ctr, 0, C_XDQ, s));
         HIPCHK(c, fill_u32(ctr + C_HIST, 0, C_N - C_HIST, s));
-        HIPCHK(c, bin_tasks(P<DpTask>(c->tasks), a, b, local ? 1 : 0, c->maxs,
+        HIPCHK(c, bin_tasks(P<DpTask>(c->tasks), a, b, c->shapes,
+                            local ? 1 : 0, c->maxs,
                             c->cfg.gap_open, c->cfg.gap_ext, lists,
                             (uint32_t)nt, ctr + C_BIN, ctr + C_HIST,
                             ctr + C_FILL,
@@ -336,8 +338,9 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
                 const int bin = 2 * cls + w;
                 tb.list = lists + (size_t)bin * nt;
                 tb.count = ctr + C_BIN + bin;
-                HIPCHK(c, launch_traceback(tb, cls, (int)((b - a + 255) / 256),
-                                           ts));
+                HIPCHK(c, launch_traceback(tb, c->shapes.shape[cls],
+                                           c->tune.tb_stage != 0,
+                                           (int)((b - a + 255) / 256), ts));
                 c->stats.launches++;
             }
             return PA_OK;
@@ -357,7 +360,8 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
                     args.sat_count = ctr + C_SAT + rcl;
                     {
                         Scope k(c, 100 + dmode * 16 + bin, s);
-                        HIPCHK(c, launch_dp(rcl, local, trace, w == 0, args,
+                        HIPCHK(c, launch_dp(c->shapes.shape[rcl], local, trace,
+                                            w == 0, args,
                                             dp_grid(c, b - a, rcl, local, trace,
                                                     w == 0),
                                             s));
@@ -371,7 +375,8 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
                     args.queue = ctr + C_SATQ + rcl;
                     {
                         Scope k(c, 100 + dmode * 16 + kBins + rcl, s);
-                        HIPCHK(c, launch_dp(rcl, local, trace, false, args,
+                        HIPCHK(c, launch_dp(c->shapes.shape[rcl], local, trace,
+                                            false, args,
                                             dp_grid(c, b - a, rcl, local, trace,
                                                     false),
                                             s));
@@ -900,8 +905,19 @@ int pa_set_tuning(pa_ctx *c, const pa_tuning *t)
         return fail(c, PA_ERR_ARG, "tuning outside its range");
     if (t->xd_coop != 0 && t->xd_coop != 1)
         return fail(c, PA_ERR_ARG, "xd_coop must be 0 or 1");
+    if (t->tb_stage != 0 && t->tb_stage != 1)
+        return fail(c, PA_ERR_ARG, "tb_stage must be 0 or 1");
+    for (int k = 0; k < 3; ++k)
+        if (t->dp_edge[k] < 1 || t->dp_edge[k] > 65535 ||
+            (k > 0 && t->dp_edge[k] <= t->dp_edge[k - 1]))
+            return fail(c, PA_ERR_ARG,
+                        "dp_edge must increase strictly within 1..65535");
+    for (int k = 0; k < 4; ++k)
+        if (t->dp_shape[k] < 0 || t->dp_shape[k] >= kShapes)
+            return fail(c, PA_ERR_ARG, "dp_shape outside pa_dp_shape");
     c->tune = *t;
-    ++c->gen;   // grids of captured graphs follow the tuning
+    c->shapes = shapes_of(*t);
+    ++c->gen;   // grids and class shapes of captured graphs follow the tuning
     return PA_OK;
 }
 
@@ -1251,6 +1267,27 @@ int pa_device_free(pa_ctx *c, void *ptr)
     return PA_OK;
 }
 
+int pa_host_alloc(pa_ctx *c, size_t bytes, void **ptr)
+{
+    if (!c || !ptr)
+        return PA_ERR_ARG;
+    hipSetDevice(c->device);
+    if (hipHostMalloc(ptr, bytes ? bytes : 1, hipHostMallocDefault) !=
+        hipSuccess) {
+        *ptr = nullptr;
+        return fail(c, PA_ERR_NOMEM, "pa_host_alloc");
+    }
+    return PA_OK;
+}
+
+int pa_host_free(pa_ctx *c, void *ptr)
+{
+    if (!c)
+        return PA_ERR_ARG;
+    HIPCHK(c, hipHostFree(ptr));
+    return PA_OK;
+}
+
 int pa_memcpy_h2d(pa_ctx *c, void *dst, const void *src, size_t bytes)
 {
     if (!c)
@@ -1336,16 +1373,18 @@ int pa_get_kernel_stats(pa_ctx *c, pa_kernel_stat *out, int cap, int *n)
             if (k < cap) {
                 pa_kernel_stat &o = out[k];
                 std::memset(&o, 0, sizeof(o));
-                // classes: 16 lanes x 4 / 8 rows, 64 lanes x 4 / 8 rows
-                static const char *cname[kClasses] = {"l16r4", "l16r8", "r4",
-                                                      "r8"};
+                // mode, row class, the class's current kernel shape
+                // (lanes x rows per lane), arithmetic
+                static const char *shname[kShapes] = {"l16r4", "l16r8",
+                                                      "l64r4", "l64r8"};
+                const int cls = b < kBins ? b / 2 : b - kBins;
+                const char *sh = shname[c->shapes.shape[cls]];
                 if (b < kBins)
-                    std::snprintf(o.name, sizeof(o.name), "%s.%s.%s",
-                                  mname[m], cname[b / 2],
-                                  b % 2 ? "i32" : "i16");
+                    std::snprintf(o.name, sizeof(o.name), "%s.c%d.%s.%s",
+                                  mname[m], cls, sh, b % 2 ? "i32" : "i16");
                 else
-                    std::snprintf(o.name, sizeof(o.name), "%s.%s.rerun",
-                                  mname[m], cname[b - kBins]);
+                    std::snprintf(o.name, sizeof(o.name), "%s.c%d.%s.rerun",
+                                  mname[m], cls, sh);
                 o.launches = ks.launches;
                 o.ms = ks.ms;
                 o.cells = b < kBins ? cells[m * kBins + b] : 0;

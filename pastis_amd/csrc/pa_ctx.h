@@ -26,6 +26,32 @@ void kmer_free(KmerState *k);
 
 }  // namespace pa
 
+namespace pa {
+inline pa_tuning default_tuning()
+{
+    pa_tuning t{};
+    t.dp_wg_per_cu = 0;
+    t.xd_wg_per_cu = 0;
+    t.xd_ring = 16;
+    t.xd_coop = 0;
+    for (int c = 0; c < 3; ++c)
+        t.dp_edge[c] = (int32_t)kDefaultShapes.edge[c];
+    for (int c = 0; c < 4; ++c)
+        t.dp_shape[c] = kDefaultShapes.shape[c];
+    t.tb_stage = 0;
+    return t;
+}
+inline DpShapes shapes_of(const pa_tuning &t)
+{
+    DpShapes sh{};
+    for (int c = 0; c < 3; ++c)
+        sh.edge[c] = (uint32_t)t.dp_edge[c];
+    for (int c = 0; c < 4; ++c)
+        sh.shape[c] = (uint8_t)t.dp_shape[c];
+    return sh;
+}
+}  // namespace pa
+
 struct pa_ctx {
     using Buf = pa::Buf;
     using Timer = pa::Timer;
@@ -92,7 +118,8 @@ struct pa_ctx {
     // captured is reallocated (grow), the sequence sets change or the launch
     // shapes change; a graph captured at another generation is recaptured.
     uint64_t gen = 0;
-    pa_tuning tune{0, 0, 16, 0};
+    pa_tuning tune = pa::default_tuning();
+    pa::DpShapes shapes = pa::kDefaultShapes;   // derived from tune
     // No event timers (pa_sw_scores_noalloc: event objects are allocated).
     bool quiet = false;
     // Cap on persistent DP workgroups per CU (0 = none); the boundary-row

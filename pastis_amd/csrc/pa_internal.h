@@ -106,36 +106,52 @@ struct TbArgs {
     const uint32_t *count;
     const uint8_t *trace;
     const uint64_t *trace_off;
-    const uint8_t *rows_per_lane; // R of the kernel that traced each task
     DpOut *out;
     int local;
 };
 
-// DP kernel class of a task by its row count (the row sequence of the task
-// runs down the lanes) and the kernel shape of each class: L lanes per task
-// (G = 64 / L tasks per wave, one per L-lane group) and R rows per lane per
-// half, so a pass covers 2RL rows and longer tasks take several passes
-// linked by a boundary row:
+// DP kernel shapes: L lanes per task (G = 64 / L tasks per wave, one per
+// L-lane group) and R rows per lane per half, so a pass covers 2RL rows and
+// longer tasks take several passes linked by a boundary row.  Every shape
+// runs any row count; the kernels are compiled for these four.
+constexpr int kShapes = 4;
+constexpr int kShapeL16R4 = 0, kShapeL16R8 = 1, kShapeL64R4 = 2,
+              kShapeL64R8 = 3;
+__host__ __device__ constexpr int shape_rows(int sh) { return (sh & 1) ? 8 : 4; }
+__host__ __device__ constexpr int shape_lanes(int sh) { return (sh & 2) ? 64 : 16; }
+
+// DP row classes: a task's class is the first c with qlen <= edge[c] (class
+// 3 takes the rest), and each class has its kernel shape.  Narrow groups
+// keep more of a wave busy on short tasks (the wavefront's fill and drain
+// take 2L steps per pass and the last pass rounds the rows up to 2RL); wide
+// ones cost fewer boundary rows, profile builds and loop overhead per cell
+// on long tasks.  The defaults (kDefaultShapes) are
 //   0: qlen <= 128: 16 lanes x 4 rows, 128 rows per pass (one pass);
 //   1: qlen <= 256: 16 lanes x 8 rows, 256 rows per pass (one pass);
-//   2: qlen <= 512: 16 lanes x 4 rows, 128 rows per pass (2-4 passes);
-//   3: longer:      16 lanes x 4 rows, 128 rows per pass.
-// Narrow groups keep more of the wave busy than 64-lane tasks do: the
-// wavefront's fill and drain take 2L steps per pass and the last pass
-// rounds the rows up to 2RL.
-__host__ __device__ inline int dp_class(uint32_t qlen)
+//   2: qlen <= 512: 64 lanes x 4 rows, 512 rows per pass (one pass);
+//   3: longer:      64 lanes x 8 rows, 1024 rows per pass.
+struct DpShapes {
+    uint32_t edge[3];
+    uint8_t shape[4];
+};
+constexpr DpShapes kDefaultShapes = {{128, 256, 512},
+                                     {kShapeL16R4, kShapeL16R8, kShapeL64R4,
+                                      kShapeL64R8}};
+
+__host__ __device__ inline int dp_class(uint32_t qlen, const DpShapes &sh)
 {
-    return qlen <= 128 ? 0 : (qlen <= 256 ? 1 : (qlen <= 512 ? 2 : 3));
+    return qlen <= sh.edge[0] ? 0
+           : qlen <= sh.edge[1] ? 1
+           : qlen <= sh.edge[2] ? 2
+                                : 3;
 }
-__host__ __device__ constexpr int class_rows(int cls) { return cls == 1 ? 8 : 4; }
-__host__ __device__ constexpr int class_lanes(int) { return 16; }
-__host__ __device__ inline int rows_per_lane(uint32_t qlen)
+__host__ __device__ inline int rows_per_lane(uint32_t qlen, const DpShapes &sh)
 {
-    return class_rows(dp_class(qlen));
+    return shape_rows(sh.shape[dp_class(qlen, sh)]);
 }
-__host__ __device__ inline int lanes_per_task(uint32_t qlen)
+__host__ __device__ inline int lanes_per_task(uint32_t qlen, const DpShapes &sh)
 {
-    return class_lanes(dp_class(qlen));
+    return shape_lanes(sh.shape[dp_class(qlen, sh)]);
 }
 
 // Trace layout of a task traced with R rows per lane and L lanes: pass after
@@ -211,10 +227,12 @@ struct XdOut {
 static_assert(sizeof(XdOut) == 40, "XdOut layout");
 
 // Launchers (k_dp.hip, k_xdrop.hip, k_misc.hip).
-hipError_t launch_dp(int cls, bool local, bool trace, bool i16,
+hipError_t launch_dp(int shape, bool local, bool trace, bool i16,
                      const DpArgs &a, int grid, hipStream_t s);
-// Traceback of the tasks of one DP class (the trace layout of its shape).
-hipError_t launch_traceback(const TbArgs &a, int cls, int grid,
+// Traceback of tasks traced by the DP kernels of one shape; stage: copy each
+// lane run (8R bytes) the walk enters into LDS with wide loads instead of
+// loading one byte per step.
+hipError_t launch_traceback(const TbArgs &a, int shape, bool stage, int grid,
                             hipStream_t s);
 // X-drop with one 16-lane row of a wave per work item (k_xdrop_coop.hip);
 // items whose band outgrows 16 columns are appended to ovf_list.
@@ -225,9 +243,9 @@ hipError_t launch_xdrop_coop(const XdTask *tasks, const uint32_t *list,
                              uint32_t *ovf_count, int grid, hipStream_t s);
 int xdrop_coop_occupancy();
 
-// Workgroups of one DP kernel variant (class as dp_class) that fit on one
+// Workgroups of one DP kernel variant (of a kernel shape) that fit on one
 // CU (occupancy).
-int dp_occupancy(int cls, bool local, bool trace, bool i16);
+int dp_occupancy(int shape, bool local, bool trace, bool i16);
 // Same for the LDS-ring X-drop kernel with `ring` slots per lane.
 int xdrop_occupancy(int ring);
 

@@ -142,6 +142,12 @@ public:
                     float tau);
     void clear_filter() { filter_on_ = false; }
 
+    // Launch-shape autotuning (pa_autotune) on the first aln_batch call, on
+    // an evenly spaced sample of that call's pairs; on by default, off when
+    // the environment sets PA_AUTOTUNE=0.  The library caches the choice per
+    // device, so later aligners on the device install it without timing.
+    void set_autotune(bool on) { autotune_ = on; }
+
     const HipAlignerStats &stats() const { return stats_; }
     pa_ctx *context() { return ctx_; }
 
@@ -153,17 +159,27 @@ private:
     void load(const SequenceSource &src);
     void check(int rc, const char *what);
     void device_reserve(uint64_t n);
+    void host_reserve(uint64_t n);
+    void tune(const pa_pair *pairs, uint64_t n);
 
     pa_ctx *ctx_ = nullptr;
     pa_config cfg_{};
     std::vector<uint32_t> rlen_, clen_;
-    std::vector<pa_pair> pairs_;
-    std::vector<pa_result> results_;
-    std::vector<uint32_t> kept_index_;
+    // Two page-locked staging slots: while the GPU aligns the pairs of one
+    // chunk, a host thread writes the previous chunk's results back into
+    // the tuples and packs the next chunk into the other slot.
+    struct Slot {
+        pa_pair *pairs = nullptr;
+        pa_result *results = nullptr;
+        uint32_t *index = nullptr;   // kept pairs (learned filter)
+        uint64_t n = 0, kept = 0;
+    } slot_[2];
+    uint64_t h_cap_ = 0;
     void *d_pairs_ = nullptr, *d_kept_ = nullptr, *d_index_ = nullptr,
          *d_results_ = nullptr;
     uint64_t d_cap_ = 0;
     bool filter_on_ = false;
+    bool autotune_ = true, tuned_ = false;
     float tau_ = 0.f;
     HipAlignerStats stats_;
 };

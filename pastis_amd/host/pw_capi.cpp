@@ -4,11 +4,13 @@
 // pw_aln_batch (src/DistPWRunner.cpp:30-160).  Used by tests and benchmarks
 // to drive the C++ layer from Python.
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <numeric>
 #include <stdexcept>
 #include <unordered_map>
 
+#include "../../include/pastis_pw.h"
 #include "pw_runner.hpp"
 
 using namespace pastis_amd;
@@ -146,6 +148,73 @@ int pw_run_block(int mode, int gap_open, int gap_ext, int klength, int xdrop,
                                         score_aln_out, stats);
     } catch (const std::exception &e) {
         std::fprintf(stderr, "pw_run_block: %s\n", e.what());
+        return PA_ERR_HIP;
+    }
+}
+
+// Times the operator path on host tuples: HipAligner::aln_batch over one
+// batch of nnz tuples built once from `pairs` (block-local r, c with
+// bl_roffset = bl_coffset = 0 on a diagonal block of the nseq sequences),
+// `warmup` untimed and `steps` timed calls.  ms_per_call receives the mean
+// wall time of a timed call (packing, staging copies, alignment, result
+// write-back), passed the pairs whose score_aln the last call wrote.
+// autotune = 0 turns the aligner's first-call autotuning off.
+int pw_bench_aln_batch(int mode, int gap_open, int gap_ext, int klength,
+                       int xdrop, int seed_count, int gap_model, int device,
+                       const char *res, const uint64_t *offs, uint64_t nseq,
+                       const pa_pair *pairs, uint64_t nnz, double cov_thr,
+                       int ani_thr, int autotune, int warmup, int steps,
+                       double *ms_per_call, uint64_t *passed)
+{
+    if (!res || !offs || (nnz && !pairs) || !ms_per_call || steps < 1 ||
+        warmup < 0)
+        return PA_ERR_ARG;
+    try {
+        params_t prm;
+        prm.aln_cov_thr = cov_thr;
+        prm.aln_ani_thr = ani_thr;
+        std::unique_ptr<HipAligner> al;
+        if (mode == 0)
+            al.reset(new HipAligner(gap_open, gap_ext, device, gap_model));
+        else
+            al.reset(new HipAligner(gap_open, gap_ext, (uint16_t)klength,
+                                    xdrop, seed_count, device, gap_model));
+        al->set_autotune(autotune != 0);
+        al->construct_seqs_bl(std::make_shared<VectorSource>(
+            std::vector<char>(res, res + offs[nseq]),
+            std::vector<uint64_t>(offs, offs + nseq + 1)));
+        std::vector<CommonKmerLoc> vals(nnz);
+        std::vector<std::tuple<uint64_t, uint64_t, CommonKmerLoc *>> tup(nnz);
+        for (uint64_t i = 0; i < nnz; ++i) {
+            CommonKmerLoc &v = vals[i];
+            v.count = pairs[i].count;
+            v.score = pairs[i].kscore;
+            v.first = {pairs[i].seed[0], pairs[i].seed[1]};
+            v.second = {pairs[i].seed[2], pairs[i].seed[3]};
+            tup[i] = std::make_tuple((uint64_t)pairs[i].r,
+                                     (uint64_t)pairs[i].c, &vals[i]);
+        }
+        double ms = 0;
+        for (int it = 0; it < warmup + steps; ++it) {
+            for (auto &v : vals)
+                v.score_aln = 0.0f;
+            const auto t0 = std::chrono::steady_clock::now();
+            al->aln_batch(tup.data(), 0, nnz, 0, 0, prm);
+            const auto t1 = std::chrono::steady_clock::now();
+            if (it >= warmup)
+                ms += std::chrono::duration<double, std::milli>(t1 - t0)
+                          .count();
+        }
+        *ms_per_call = ms / steps;
+        if (passed) {
+            uint64_t k = 0;
+            for (const auto &v : vals)
+                k += v.score_aln != 0.0f;
+            *passed = k;
+        }
+        return PA_OK;
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "pw_bench_aln_batch: %s\n", e.what());
         return PA_ERR_HIP;
     }
 }

@@ -31,6 +31,9 @@ def lib():
                                                 u64, u64, u64] + [vp] * 5 +
                                    [i32, C.c_float, u64, C.c_double, i32,
                                     vp, vp, vp, C.c_float, vp, vp])
+        L.pw_bench_aln_batch.argtypes = ([i32] * 8 + [vp, vp, u64, vp, u64,
+                                                      C.c_double, i32, i32,
+                                                      i32, i32, vp, vp])
         _pw = L
     return _pw
 
@@ -87,3 +90,27 @@ def run_block(mode, codes, offsets, rows, cols, count, score, seeds=None,
                             _p(out), _p(st))
     _lib.check(None, rc, "pw_run_block")
     return out, dict(zip(STATS, (int(v) for v in st)))
+
+
+def bench_aln_batch(mode, codes, offsets, pairs, gap_open=-11, gap_ext=-2,
+                    klength=6, xdrop=49, seed_count=2, gap_model="as_called",
+                    cov_thr=0.7, ani_thr=30, autotune=True, warmup=1,
+                    steps=3, device=0):
+    """Wall time of HipAligner::aln_batch on host tuples made from `pairs`
+    (PAIR_DTYPE, block-local indices of a diagonal block): returns
+    (ms per call, pairs passed)."""
+    res = np.ascontiguousarray(ascii_of(codes))
+    offs = np.ascontiguousarray(offsets, np.uint64)
+    pairs = np.ascontiguousarray(pairs, _lib.PAIR_DTYPE)
+    gm = {"as_called": _lib.PA_GAP_AS_CALLED,
+          "as_documented": _lib.PA_GAP_AS_DOCUMENTED}[gap_model]
+    ms = C.c_double(0)
+    passed = C.c_uint64(0)
+    rc = lib().pw_bench_aln_batch(mode, gap_open, gap_ext, klength, xdrop,
+                                  seed_count, gm, device, _p(res), _p(offs),
+                                  len(offs) - 1, _p(pairs), len(pairs),
+                                  cov_thr, ani_thr, 1 if autotune else 0,
+                                  warmup, steps, C.byref(ms),
+                                  C.byref(passed))
+    _lib.check(None, rc, "pw_bench_aln_batch")
+    return ms.value, passed.value

@@ -112,6 +112,19 @@ int main(int argc, char **argv)
     pa_tuning t;
     pa_get_tuning(c, &t);
     t.xd_coop = h[7];
+    // EMU_SHAPES = "s0,s1,s2,s3,e0,e1,e2,stage": row-class kernel shapes,
+    // class edges and traceback staging
+    if (const char *e = std::getenv("EMU_SHAPES")) {
+        int v[8];
+        if (std::sscanf(e, "%d,%d,%d,%d,%d,%d,%d,%d", &v[0], &v[1], &v[2],
+                        &v[3], &v[4], &v[5], &v[6], &v[7]) != 8)
+            return 2;
+        for (int k = 0; k < 4; ++k)
+            t.dp_shape[k] = v[k];
+        for (int k = 0; k < 3; ++k)
+            t.dp_edge[k] = v[4 + k];
+        t.tb_stage = v[7];
+    }
     if ((rc = pa_set_tuning(c, &t)))
         return fail("pa_set_tuning", rc, c);
     if ((rc = pa_load_seqs(c, PA_SIDE_BOTH, codes.data(), offs.data(), nseq,

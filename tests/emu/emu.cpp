@@ -91,6 +91,14 @@ hipError_t hipFree(void *p)
     std::free(p);
     return hipSuccess;
 }
+hipError_t hipHostMalloc(void **p, size_t n, unsigned)
+{
+    return hipMalloc(p, n);
+}
+hipError_t hipHostFree(void *p)
+{
+    return hipFree(p);
+}
 hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind)
 {
     std::memmove(d, s, n);

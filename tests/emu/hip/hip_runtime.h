@@ -211,7 +211,14 @@ inline unsigned emu_alignbit(unsigned a, unsigned b, unsigned s)
 
 // runtime
 hipError_t hipMalloc(void **p, size_t n);
+template <class T> inline hipError_t hipMalloc(T **p, size_t n)
+{
+    return hipMalloc((void **)p, n);
+}
 hipError_t hipFree(void *p);
+#define hipHostMallocDefault 0u
+hipError_t hipHostMalloc(void **p, size_t n, unsigned flags);
+hipError_t hipHostFree(void *p);
 hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind);
 hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, hipMemcpyKind,
                           hipStream_t);

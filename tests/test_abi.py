@@ -32,7 +32,7 @@ def test_operator_layer_exports_its_entry_point():
     text = open(os.path.join(ROOT, "include", "pastis_pw.h")).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     decl = sorted(set(re.findall(r"\b(pw_[a-z0-9_]+)\s*\(", text)))
-    assert decl == ["pw_run_block"]
+    assert decl == ["pw_bench_aln_batch", "pw_run_block"]
     from pastis_amd import pwalign
     L = pwalign.lib()
     for name in decl:

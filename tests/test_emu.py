@@ -36,7 +36,7 @@ def driver():
 
 
 def run(driver, tmp_path, w, pairs, mode, xdrop=0, coop=0,
-        gap_model=0):
+        gap_model=0, shapes=None):
     inp = tmp_path / "in.bin"
     outp = tmp_path / "out.bin"
     hdr = np.array([mode, -11, -2, 6, xdrop, 2, gap_model, coop], np.int32)
@@ -49,6 +49,8 @@ def run(driver, tmp_path, w, pairs, mode, xdrop=0, coop=0,
         f.write(np.ascontiguousarray(pairs, PAIR_DTYPE).tobytes())
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1",
                UBSAN_OPTIONS="print_stacktrace=1")
+    if shapes is not None:
+        env["EMU_SHAPES"] = ",".join(str(v) for v in shapes)
     r = subprocess.run([driver, str(inp), str(outp)], env=env,
                        capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stderr[-4000:]
@@ -65,9 +67,16 @@ def run(driver, tmp_path, w, pairs, mode, xdrop=0, coop=0,
     return res, ext, cells
 
 
-def test_emulated_sfa_under_sanitizers(driver, tmp_path):
+# Non-default launch shapes: 64-lane kernels on short rows and 16-lane
+# multi-pass kernels (128 rows per pass) on long ones, class edges that send
+# most tasks through several passes, LDS-staged tracebacks.
+SHAPES = [None, (3, 2, 1, 0, 16, 40, 100, 1)]
+
+
+@pytest.mark.parametrize("shapes", SHAPES)
+def test_emulated_sfa_under_sanitizers(driver, tmp_path, shapes):
     w = small_config1(n_seq=20, n_pairs=20, seed=71)
-    got, _, _ = run(driver, tmp_path, w, w.pairs, 0)
+    got, _, _ = run(driver, tmp_path, w, w.pairs, 0, shapes=shapes)
     ref = orc.sfa_batch(w.codes, w.offsets, w.pairs["r"], w.pairs["c"],
                         oracle_params("as_called"))
     for gf, rf in [("score", "score"), ("r_end", "v_end"), ("c_end", "h_end"),
@@ -79,11 +88,12 @@ def test_emulated_sfa_under_sanitizers(driver, tmp_path):
     assert np.array_equal(got["score_aln"], ref["score_aln"])
 
 
-@pytest.mark.parametrize("coop", [0, 1])
-def test_emulated_sxa_under_sanitizers(driver, tmp_path, coop):
+@pytest.mark.parametrize("coop,shapes", [(0, None), (1, None),
+                                         (0, SHAPES[1])])
+def test_emulated_sxa_under_sanitizers(driver, tmp_path, coop, shapes):
     w = small_config2(n_seq=20, n_pairs=16, seed=72)
     got, ext, cells = run(driver, tmp_path, w, w.pairs, 1, xdrop=49,
-                          coop=coop)
+                          coop=coop, shapes=shapes)
     ref = orc.sxa_batch(w.codes, w.offsets, w.pairs["r"], w.pairs["c"],
                         w.pairs["seed"], oracle_params("as_called"))
     for gf in ("score", "matches", "cols", "gap_opens"):
