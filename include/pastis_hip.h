@@ -437,6 +437,11 @@ typedef struct pa_tuning {
     int32_t tb_stage;       /* 1: the traceback copies each lane run of the
                                trace it enters into LDS with wide loads;
                                0: one byte load per step */
+    int32_t xd_wave;        /* 1 (default): X-drop items whose band outgrows
+                               the per-lane kernel's rings continue with one
+                               wave per item (bands up to 1024 columns), the
+                               rest with global-memory rings; 0: straight to
+                               the global-memory rings, one lane per item */
 } pa_tuning;
 
 int pa_get_tuning(struct pa_ctx *ctx, pa_tuning *out);

@@ -83,7 +83,9 @@ enum {
     C_ERR = 29,         // validation errors
     C_XDLONG = 30,      // X-drop items left by the 16-lane kernel
     C_XDLONGQ = 31,     // their queue head
-    C_HIST = 32,        // kBins x kBuckets histogram of the counting sort
+    C_XDWOVF = 32,      // X-drop items left by the wave-per-item kernel
+    C_XDWOVFQ = 33,     // their queue head
+    C_HIST = 36,        // kBins x kBuckets histogram of the counting sort
     C_FILL = C_HIST + kBins * kBuckets,   // its running slots
     C_N = C_FILL + kBins * kBuckets
 };
@@ -526,8 +528,25 @@ int run_xdrop(pa_ctx *c, const pa_pair *d_pairs, uint64_t n, hipStream_t s)
             c->stats.launches++;
         }
     }
+    // items whose band outgrew the LDS rings, one wave per item; what
+    // outgrows its rings too is left in `lng` for the global-ring kernel
+    // (the per-lane kernel is done reading `lng` by then)
+    uint32_t *last = ovf, *last_n = ctr + C_XDOVF, *last_q = ctr + C_XDOVFQ;
+    if (c->tune.xd_wave) {
+        Scope t(c, 1, s);
+        HIPCHK(c, launch_xdrop_wave(P<XdTask>(c->xdt), ovf, ctr + C_XDOVF,
+                                    ctr + C_XDOVFQ, P<int8_t>(c->dmat),
+                                    c->cfg.xdrop_gap, c->cfg.xdrop,
+                                    c->cfg.seed_len, P<XdOut>(c->xdo), lng,
+                                    ctr + C_XDWOVF,
+                                    c->ncu * xdrop_wave_occupancy(), s));
+        c->stats.launches++;
+        last = lng;
+        last_n = ctr + C_XDWOVF;
+        last_q = ctr + C_XDWOVFQ;
+    }
     uint32_t novf = 0;
-    HIPCHK(c, hipMemcpyAsync(&novf, ctr + C_XDOVF, 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipMemcpyAsync(&novf, last_n, 4, hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipStreamSynchronize(s));
     if (novf) {
         // rings in global scratch holding a whole segment (power of two)
@@ -544,11 +563,13 @@ int run_xdrop(pa_ctx *c, const pa_pair *d_pairs, uint64_t n, hipStream_t s)
             return rc;
         {
             Scope t(c, 1, s);
-            HIPCHK(c, launch_xdrop(P<XdTask>(c->xdt), ovf, ctr + C_XDOVF,
-                                   ctr + C_XDOVFQ, P<int8_t>(c->dmat),
-                                   c->cfg.xdrop_gap, c->cfg.xdrop,
-                                   c->cfg.seed_len, P<int32_t>(wide), cap,
-                                   P<XdOut>(c->xdo), lng, ctr + C_XDDUMMY,
+            // its overflow list (never written: the rings hold a whole
+            // segment) is the list the previous kernel consumed
+            HIPCHK(c, launch_xdrop(P<XdTask>(c->xdt), last, last_n, last_q,
+                                   P<int8_t>(c->dmat), c->cfg.xdrop_gap,
+                                   c->cfg.xdrop, c->cfg.seed_len,
+                                   P<int32_t>(wide), cap, P<XdOut>(c->xdo),
+                                   last == ovf ? lng : ovf, ctr + C_XDDUMMY,
                                    (int)blocks2, 0, s));
             c->stats.launches++;
         }
@@ -907,6 +928,8 @@ int pa_set_tuning(pa_ctx *c, const pa_tuning *t)
         return fail(c, PA_ERR_ARG, "xd_coop must be 0 or 1");
     if (t->tb_stage != 0 && t->tb_stage != 1)
         return fail(c, PA_ERR_ARG, "tb_stage must be 0 or 1");
+    if (t->xd_wave != 0 && t->xd_wave != 1)
+        return fail(c, PA_ERR_ARG, "xd_wave must be 0 or 1");
     for (int k = 0; k < 3; ++k)
         if (t->dp_edge[k] < 1 || t->dp_edge[k] > 65535 ||
             (k > 0 && t->dp_edge[k] <= t->dp_edge[k - 1]))

@@ -39,6 +39,7 @@ inline pa_tuning default_tuning()
     for (int c = 0; c < 4; ++c)
         t.dp_shape[c] = kDefaultShapes.shape[c];
     t.tb_stage = 0;
+    t.xd_wave = 1;
     return t;
 }
 inline DpShapes shapes_of(const pa_tuning &t)

@@ -11,12 +11,19 @@
 //   3. the kernel shape of each row class, by that class's DP time plus the
 //      traceback time (a shape also fixes the trace layout);
 //   4. persistent DP workgroups per CU, by DP time;
-//   5. traceback staging, by traceback time.
+//   5. traceback staging, by traceback time;
+//   6. X-drop items outgrowing the per-lane rings: wave per item or global
+//      rings, by X-drop time.
 // A candidate replaces the incumbent only when it is faster by more than
-// kMinGain, so timing noise does not flip choices.  Results never depend on
-// the tuning; only the speed does.
+// kMinGain, so timing noise does not flip choices.  A sample can mislead on
+// heavy-tailed length mixes (a few long tasks set a kernel's time), so when
+// the sample is smaller than the caller's set, the choice is checked against
+// the starting tuning on up to kValidate evenly spaced pairs of the whole
+// set and kept only if it is faster there too.  Results never depend on the
+// tuning; only the speed does.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -31,10 +38,12 @@ namespace {
 
 constexpr double kMinGain = 0.01;
 constexpr uint64_t kDefaultSample = 262144;
+constexpr uint64_t kValidate = 1u << 22;
 constexpr int kReps = 2;
 
 struct Times {
     double dp = 0, cls[kClasses] = {0, 0, 0, 0}, trace = 0, xdrop = 0;
+    double total() const { return dp + trace + xdrop; }
 };
 
 std::mutex g_mu;
@@ -202,10 +211,32 @@ int search(Tuner &tu, bool xdrop, pa_tuning &t)
         Times tm;
         if ((rc = tu.measure(cand, &tm)))
             return rc;
-        if (faster(tm.trace, cur.trace))
+        if (faster(tm.trace, cur.trace)) {
+            t = cand;
+            cur = tm;
+        }
+    }
+
+    // 6. wide X-drop bands
+    if (xdrop) {
+        pa_tuning cand = t;
+        cand.xd_wave = 1 - t.xd_wave;
+        Times tm;
+        if ((rc = tu.measure(cand, &tm)))
+            return rc;
+        if (faster(tm.xdrop, cur.xdrop))
             t = cand;
     }
     return PA_OK;
+}
+
+// Evenly spaced pairs [0, m) of src[0, n) into dst.
+hipError_t gather(const pa_pair *src, uint64_t n, uint64_t m, pa_pair *dst,
+                  hipStream_t s)
+{
+    hipLaunchKernelGGL(k_gather_pairs, dim3((unsigned)((m + 255) / 256)),
+                       dim3(256), 0, s, src, n, m, dst);
+    return hipStreamSynchronize(s);
 }
 
 }  // namespace
@@ -237,30 +268,41 @@ extern "C" int pa_autotune(pa_ctx *c, const pa_pair *d_pairs, uint64_t n,
     }
     HIPCHK(c, hipStreamSynchronize(c->st));
     const uint64_t m = std::min(n, sample ? sample : kDefaultSample);
-    const size_t out_bytes =
-        m * (target == PA_TUNE_SW_SCORES ? sizeof(pa_score)
-                                         : sizeof(pa_result));
+    const uint64_t mv = m < n ? std::min(n, std::max(m, kValidate)) : 0;
+    const uint64_t cap = std::max(m, mv);
+    const size_t res_size = target == PA_TUNE_SW_SCORES ? sizeof(pa_score)
+                                                        : sizeof(pa_result);
     pa_pair *dp = nullptr;
     void *dout = nullptr;
-    if (hipMalloc(&dp, m * sizeof(pa_pair)) != hipSuccess ||
-        hipMalloc(&dout, out_bytes) != hipSuccess) {
+    if (hipMalloc(&dp, cap * sizeof(pa_pair)) != hipSuccess ||
+        hipMalloc(&dout, cap * res_size) != hipSuccess) {
         hipFree(dp);
         return fail(c, PA_ERR_NOMEM, "pa_autotune sample buffers");
     }
-    hipLaunchKernelGGL(k_gather_pairs, dim3((unsigned)((m + 255) / 256)),
-                       dim3(256), 0, c->st, d_pairs, n, m, dp);
-    hipError_t e = hipStreamSynchronize(c->st);
     const pa_tuning start = c->tune;
     pa_tuning t = start;
     int rc = PA_OK;
+    hipError_t e = gather(d_pairs, n, m, dp, c->st);
+    const bool xdrop = target == PA_TUNE_ALIGN && c->cfg.mode == PA_MODE_XDROP;
     if (e != hipSuccess) {
         rc = fail(c, PA_ERR_HIP,
                   std::string("pa_autotune gather: ") + hipGetErrorString(e));
     } else {
         Tuner tu(c, dp, m, target, dout);
-        const bool xdrop =
-            target == PA_TUNE_ALIGN && c->cfg.mode == PA_MODE_XDROP;
         rc = search(tu, xdrop, t);
+    }
+    if (!rc && mv && std::memcmp(&t, &start, sizeof(t)) != 0) {
+        // the choice against the starting tuning on a larger share of the set
+        if ((e = gather(d_pairs, n, mv, dp, c->st)) != hipSuccess) {
+            rc = fail(c, PA_ERR_HIP, std::string("pa_autotune gather: ") +
+                                         hipGetErrorString(e));
+        } else {
+            Tuner tv(c, dp, mv, target, dout);
+            Times a, b;
+            if (!(rc = tv.measure(start, &a)) && !(rc = tv.measure(t, &b)) &&
+                !faster(b.total(), a.total()))
+                t = start;
+        }
     }
     hipFree(dp);
     hipFree(dout);

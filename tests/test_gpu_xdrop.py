@@ -99,16 +99,18 @@ def test_seed_out_of_range_is_an_error():
 
 
 def test_results_do_not_depend_on_tuning():
-    """Launch shapes (ring slots, workgroups per CU) change speed only."""
+    """Launch shapes (ring slots, workgroups per CU, the kernel that takes
+    the items whose band outgrows the per-lane rings) change speed only."""
     w = small_config2(n_pairs=800, seed=17)
     base = None
     for model in ("as_called", "as_documented"):
         base = None
-        for ring, dp, xd, pl in [(16, 0, 0, 0), (32, 0, 0, 1), (16, 1, 1, 1),
-                                 (32, 3, 2, 0), (16, 0, 0, 1)]:
+        for ring, dp, xd, pl, wv in [(16, 0, 0, 0, 1), (32, 0, 0, 1, 1),
+                                     (16, 1, 1, 1, 0), (32, 3, 2, 0, 1),
+                                     (16, 0, 0, 1, 0), (32, 0, 0, 1, 0)]:
             al = make_aligner(model)
             al.set_tuning(xd_ring=ring, dp_wg_per_cu=dp, xd_wg_per_cu=xd,
-                          xd_coop=1 - pl)
+                          xd_coop=1 - pl, xd_wave=wv)
             got_t = al.tuning()
             assert {k: got_t[k] for k in ("dp_wg_per_cu", "xd_wg_per_cu",
                                           "xd_ring", "xd_coop")} == dict(
@@ -120,7 +122,7 @@ def test_results_do_not_depend_on_tuning():
             cells = al.stats()["xdrop_cells"]
             if base is None:
                 base = (got, cells)
-            assert np.array_equal(got, base[0]), (model, ring, dp, xd, pl)
+            assert np.array_equal(got, base[0]), (model, ring, dp, xd, pl, wv)
             assert cells == base[1]
     al = make_aligner("as_called")
     with pytest.raises(_lib.PaError):
