@@ -2,11 +2,13 @@
 
     python -m pastis_amd.rocprof_summary trace DB [--top N]
     python -m pastis_amd.rocprof_summary counters DB COUNTER [--scale S]
+    python -m pastis_amd.rocprof_summary table DB COUNTER [COUNTER ...]
 
 `trace` lists kernels by total duration (calls, total/average ms, share).
 `counters` lists the per-dispatch average of one counter per kernel,
 multiplied by `--scale` (the calibration factor measured on a copy kernel of
-known size).
+known size).  `table` lists the per-dispatch averages of several counters
+collected in one pass, one column per counter.
 """
 import argparse
 import re
@@ -48,6 +50,26 @@ def counters(db, counter, scale):
     return "\n".join(out)
 
 
+def table(db, names, top):
+    c = sqlite3.connect(db)
+    rows = list(c.execute(
+        "select kernel_name, counter_name, count(*), avg(value) from "
+        "counters_collection group by kernel_name, counter_name"))
+    per = {}
+    disp = {}
+    for k, n, cnt, avg in rows:
+        per.setdefault(k, {})[n] = avg
+        disp[k] = max(disp.get(k, 0), cnt)
+    order = sorted(per, key=lambda k: -per[k].get(names[0], 0.0))[:top]
+    out = ["%-60s %5s " % ("kernel", "disp") +
+           " ".join("%18s" % n[:18] for n in names)]
+    for k in order:
+        out.append("%-60s %5d " % (short(k, 60), disp[k]) +
+                   " ".join("%18.1f" % per[k].get(n, float("nan"))
+                            for n in names))
+    return "\n".join(out)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -58,9 +80,15 @@ def main(argv=None):
     k.add_argument("db")
     k.add_argument("counter")
     k.add_argument("--scale", type=float, default=1.0)
+    tb = sub.add_parser("table")
+    tb.add_argument("db")
+    tb.add_argument("names", nargs="+")
+    tb.add_argument("--top", type=int, default=20)
     a = ap.parse_args(argv)
     if a.cmd == "trace":
         print(trace(a.db, a.top))
+    elif a.cmd == "table":
+        print(table(a.db, a.names, a.top))
     else:
         print(counters(a.db, a.counter, a.scale))
 
