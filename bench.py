@@ -28,6 +28,7 @@ headline instead.
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -62,12 +63,57 @@ def host_cores():
 FILTER_FLOP_PER_PAIR = 2 * (16 * 64 + 64 * 64 + 64)
 
 
-def roofline(kstats, st, step_ms_total, tuning=None):
+# rocprof FETCH_SIZE / WRITE_SIZE summaries (KiB per dispatch) of a
+# workload, with their copy-kernel calibration factors
+TRAFFIC_PROFILES = {
+    ("xdrop-lognormal", "as_called", 1.0):
+        ("profiles/xdrop_5M_pairs_hbm_traffic.md", 2.0, 1.0),
+}
+
+
+def rocprof_name(kernel):
+    """k_dp<nwt.c3.l64r8.i16> -> void pa::k_dp<8, 64, false, true, true>"""
+    m = re.match(r"k_dp<(sw|swt|nwt)\.c\d\.l(\d+)r(\d+)\.(i16|i32)>", kernel)
+    if not m:
+        return None
+    local, trace = {"sw": ("true", "false"), "swt": ("true", "true"),
+                    "nwt": ("false", "true")}[m.group(1)]
+    return "void pa::k_dp<%s, %s, %s, %s, %s>" % (
+        m.group(3), m.group(2), local, trace,
+        "true" if m.group(4) == "i16" else "false")
+
+
+def profiled_traffic(kernel, key):
+    """HBM bytes per launch of `kernel` from the committed rocprof summary
+    of this workload (FETCH_SIZE and WRITE_SIZE tables, calibrated), or
+    None when the workload or kernel was not profiled."""
+    prof = TRAFFIC_PROFILES.get(key)
+    name = rocprof_name(kernel)
+    if not prof or not name:
+        return None, None
+    path, ff, fw = prof
+    full = os.path.join(os.path.dirname(os.path.abspath(__file__)), path)
+    if not os.path.exists(full):
+        return None, None
+    vals = {}
+    for line in open(full):
+        if line.startswith(name):
+            parts = line.split()
+            vals.setdefault("fetch" if "fetch" not in vals else "write",
+                            float(parts[-1]))
+    if len(vals) != 2:
+        return None, None
+    return int((vals["fetch"] * ff + vals["write"] * fw) * 1024), path
+
+
+def roofline(kstats, st, step_ms_total, tuning=None, traffic_key=None):
     """Dominant kernel by device time: algorithmic ops / launch time.
 
     Candidates are the DP kernel classes (k_dp instantiations) and the X-drop
     extension kernel; ops per cell follow the fixed table (Gotoh local 10,
-    global on the infix 9, X-drop 7 per evaluated cell)."""
+    global on the infix 9, X-drop 7 per evaluated cell).  traffic: HBM bytes
+    per launch from the committed rocprof summary of the workload
+    (traffic_key = (workload, gap model, scale)), when there is one."""
     cands = []
     for k in kstats:
         mode = k["name"].split(".")[0]
@@ -87,9 +133,11 @@ def roofline(kstats, st, step_ms_total, tuning=None):
         return None
     top = max(cands, key=lambda k: k["ms"])
     tops = top["ops"] * top["cells"] / (top["ms"] * 1e-3) / 1e12
+    traffic, traffic_src = profiled_traffic(top["kernel"], traffic_key)
     return {"bound": "valu", "kernel": top["kernel"],
             "achieved": round(tops, 3), "peak": top["peak"], "unit": "Tops/s",
-            "frac": round(tops / top["peak"], 4), "traffic": None,
+            "frac": round(tops / top["peak"], 4), "traffic": traffic,
+            "traffic_source": traffic_src,
             "ops_per_cell": top["ops"], "cells": int(top["cells"]),
             "kernel_ms": round(top["ms"], 3),
             "launches": top["launches"],
@@ -545,6 +593,9 @@ def main():
                          "(HBM counter calibration) and exit")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes")
+    ap.add_argument("--set-tuning", default="",
+                    help="launch shapes to install, e.g. tb_stage=1,"
+                         "dp_shape=0:1:2:3 (profiling runs)")
     ap.add_argument("--no-operator", action="store_true",
                     help="skip timing HipAligner::aln_batch on host tuples")
     ap.add_argument("--no-filter", action="store_true",
@@ -618,6 +669,14 @@ def main():
         from pastis_amd import autotune
         _, tune_s = autotune.tune(al, (d_pairs, n),
                                   xdrop=args.workload == "xdrop", log=log)
+    if args.set_tuning:
+        # explicit launch shapes (after any autotuning): k=v,k=v with
+        # dp_edge / dp_shape given as a:b:c
+        kw = {}
+        for item in args.set_tuning.split(","):
+            k, v = item.split("=")
+            kw[k] = [int(x) for x in v.split(":")] if ":" in v else int(v)
+        al.set_tuning(**kw)
     tuning = al.tuning()
 
     for i in range(args.warmup):
@@ -664,7 +723,8 @@ def main():
         "xdrop_cells_per_step": int(st["xdrop_cells"] / args.steps),
         "passing_pairs": int(((out["flags"] & 1) != 0).sum())
         if args.workload == "xdrop" else None,
-        "roofline": roofline(ks, st, elapsed * 1e3, tuning),
+        "roofline": roofline(ks, st, elapsed * 1e3, tuning,
+                             (w.name, args.gap_model, args.scale)),
         "device_ms_per_step": {k: round(st[k] / args.steps, 3) for k in
                                ("ms_sw", "ms_xdrop", "ms_nw", "ms_trace")},
         "cpu_baseline": None,

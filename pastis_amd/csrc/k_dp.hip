@@ -479,16 +479,17 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
             if (L == 64)   // one task per wave: keep it in a scalar register
                 seg_skip = __builtin_amdgcn_readfirstlane(seg_skip);
             // Column residues and (in later passes) the boundary row reach
-            // lane 0 through v_readlane from registers filled 64 columns at a
-            // time: lane l holds column base + l, and the next block is
-            // loaded one block ahead, so no step waits on global memory.
-            // The boundary row of this pass is written 128 steps after it is
+            // lane 0 through v_readlane / a shuffle from registers that are
+            // filled a block ahead: lane l holds column base + l (64-lane
+            // tasks) or columns base + 4l .. base + 4l + 3 (16-lane tasks)
+            // of the residues, and column base + l of the boundary row, with
+            // base = 1 (mod block).  The blocks switch at steps t = 0 (mod L)
+            // outside the step loop, so no step waits on global memory.  The
+            // boundary row of this pass is written 128 steps after it is
             // read, so the look-ahead never reads a value of this pass.
             auto res_at = [&](int j) {   // residue of column j (1-based)
                 return j <= tlen ? (int)ts[j - 1] : kSentinel;
             };
-            // 64-lane tasks: lane l holds column base + l; 16-lane tasks:
-            // lane l of a slot holds columns base + 4l .. base + 4l + 3
             auto res_blk = [&](int base) {   // block of 64 columns from base
                 if (L == 64)
                     return res_at(base + lane);
@@ -497,18 +498,15 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
                        (res_at(c + 2) << 16) | (res_at(c + 3) << 24);
             };
             int rc_cur = res_blk(1), rc_nxt = res_blk(65);
-            // Boundary row of a later pass: blocks of L columns, lane gl of
-            // a slot holding column base + gl; the slot's first lane takes
-            // column jn with readlane (L = 64) or a shuffle within the slot.
             int bh_cur = 0, bf_cur = NEGF, bh_nxt = 0, bf_nxt = NEGF;
             if (p > 0) {
-                if (gl <= tlen) {
-                    bh_cur = bnd[2 * gl];
-                    bf_cur = bnd[2 * gl + 1];
+                if (gl + 1 <= tlen) {
+                    bh_cur = bnd[2 * (gl + 1)];
+                    bf_cur = bnd[2 * (gl + 1) + 1];
                 }
-                if (L + gl <= tlen) {
-                    bh_nxt = bnd[2 * (L + gl)];
-                    bf_nxt = bnd[2 * (L + gl) + 1];
+                if (L + gl + 1 <= tlen) {
+                    bh_nxt = bnd[2 * (L + gl + 1)];
+                    bf_nxt = bnd[2 * (L + gl + 1) + 1];
                 }
             }
             // Column residues of step tn from those of step tn-1: lane 0
@@ -517,10 +515,6 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
             // branch surrounds the DPP move.
             auto next_res = [&](int tn) {
                 const int ri = (tn - 1) & 63;
-                if (ri == 0 && tn > 1) {
-                    rc_cur = rc_nxt;
-                    rc_nxt = res_blk(tn + 64);
-                }
                 int c0;
                 if (L == 64) {
                     c0 = __builtin_amdgcn_readlane(rc_cur, ri);
@@ -537,42 +531,32 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
             };
             // residues and profile words of step 1
             next_res(1);
-            uint32_t lw_n[R / 4], hw_n[R / 4];
-            ProfWords<R>::load(prof + c_lo * ROWS + 2 * R * gl, lw_n);
-            ProfWords<R>::load(prof + c_hi * ROWS + 2 * R * gl + R, hw_n);
+            uint32_t lwA[R / 4], hwA[R / 4], lwB[R / 4], hwB[R / 4];
+            ProfWords<R>::load(prof + c_lo * ROWS + 2 * R * gl, lwA);
+            ProfWords<R>::load(prof + c_hi * ROWS + 2 * R * gl + R, hwA);
             // The step loop, specialised for the first pass (boundary row
-            // from the gap model) and later passes (boundary row from
-            // the previous pass).
+            // from the gap model) and later passes (boundary row from the
+            // previous pass).  Steps run in blocks of L between the block
+            // switches, two at a time with the profile words of the next
+            // step loaded into the other buffer, so that no register copy at
+            // the loop's back edge waits for a load.
             auto run_steps = [&](auto first_pass) {
                 constexpr bool FIRST = decltype(first_pass)::value;
-                for (int t = 1; t <= last_step; ++t) {
-                    // profile words of this step (loaded one step ahead);
-                    // then the residues of step t+1 and their profile words,
-                    // so that the LDS latency overlaps this step's work
-                    uint32_t lw[R / 4], hw[R / 4];
-#pragma unroll
-                    for (int k = 0; k < R / 4; ++k) {
-                        lw[k] = lw_n[k];
-                        hw[k] = hw_n[k];
-                    }
+                // one step t: lw / hw hold its profile words; the words of
+                // step t+1 are loaded into lwn / hwn
+                auto step = [&](int t, const uint32_t *lw, const uint32_t *hw,
+                                uint32_t *lwn, uint32_t *hwn) {
                     next_res(t + 1);
-                    ProfWords<R>::load(prof + c_lo * ROWS + 2 * R * gl, lw_n);
+                    ProfWords<R>::load(prof + c_lo * ROWS + 2 * R * gl, lwn);
                     ProfWords<R>::load(prof + c_hi * ROWS + 2 * R * gl + R,
-                                       hw_n);
+                                       hwn);
                     int nbh = 0, nbf = NEGF;  // boundary for lane 0 at step t+1
                     {
                         const int jn = t + 1;
                         if constexpr (FIRST) {
                             nbh = LOCAL ? 0 : open + (jn - 1) * gmax;
                         } else {
-                            const int bi = jn & (L - 1);
-                            if (bi == 0) {   // wave-uniform
-                                bh_cur = bh_nxt;
-                                bf_cur = bf_nxt;
-                                const int jb = jn + L + gl;
-                                bh_nxt = jb <= tlen ? bnd[2 * jb] : 0;
-                                bf_nxt = jb <= tlen ? bnd[2 * jb + 1] : NEGF;
-                            }
+                            const int bi = (jn - 1) & (L - 1);
                             if (L == 64) {
                                 if (jn <= tlen) {
                                     nbh = __builtin_amdgcn_readlane(bh_cur, bi);
@@ -683,6 +667,37 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
                         trp += R;
                         if (t % (int)kTraceSeg == 0)   // wave-uniform
                             trp += seg_skip;
+                    }
+                };
+                for (int tb = 0; tb <= last_step; tb += L) {
+                    if (tb > 0) {
+                        // block switch before step tb (tb = 0 mod L): the
+                        // residues every 64 steps, the boundary row every L
+                        if ((tb & 63) == 0) {
+                            rc_cur = rc_nxt;
+                            rc_nxt = res_blk(tb + 65);
+                        }
+                        if constexpr (!FIRST) {
+                            bh_cur = bh_nxt;
+                            bf_cur = bf_nxt;
+                            const int jb = tb + 1 + L + gl;
+                            bh_nxt = jb <= tlen ? bnd[2 * jb] : 0;
+                            bf_nxt = jb <= tlen ? bnd[2 * jb + 1] : NEGF;
+                        }
+                    }
+                    const int te = min(last_step, tb + L - 1);
+                    int t = tb > 0 ? tb : 1;
+                    for (; t < te; t += 2) {
+                        step(t, lwA, hwA, lwB, hwB);
+                        step(t + 1, lwB, hwB, lwA, hwA);
+                    }
+                    if (t == te) {
+                        step(t, lwA, hwA, lwB, hwB);
+#pragma unroll
+                        for (int k = 0; k < R / 4; ++k) {
+                            lwA[k] = lwB[k];
+                            hwA[k] = hwB[k];
+                        }
                     }
                 }
             };
