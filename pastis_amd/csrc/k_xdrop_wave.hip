@@ -1,20 +1,27 @@
-// k_xdrop_wave.hip -- gapped X-drop seed extension with one wave per work
-// item, for the items whose band outgrows the per-lane kernel's LDS rings
-// (k_xdrop.hip).  Same algorithm, step for step, as k_xdrop_pl and the
-// oracle's or_xdrop_one (SeqAn's one-direction GappedXDrop as called at
+// k_xdrop_wave.hip -- gapped X-drop seed extension with one 16-lane row of a
+// wave per work item (four items per wave), for the items whose band
+// outgrows the per-lane kernel's LDS rings (k_xdrop.hip).  Same algorithm,
+// step for step, as k_xdrop_pl and the oracle's or_xdrop_one (SeqAn's
+// one-direction GappedXDrop as called at
 // src/align/SeqanXdropAligner.cpp:281-283), so extents and evaluated-cell
 // counts are bit-identical.
 //
-// The cells of one anti-diagonal depend only on the two before it, so the
-// 64 lanes of a wave evaluate 64 consecutive columns of the band at once,
-// chunk after chunk.  The three live anti-diagonals sit in LDS rings of
-// kCap columns indexed by absolute column (the band of an item never spans
-// more than kCap columns here; a wider one is handed on to the global-ring
-// kernel through the overflow list); the residues of both segments are
-// staged in LDS when they fit.  The band update is a pair of ballots (the
-// first column from the left and the last from the right that stay live)
-// and the anti-diagonal's best score a wave maximum.  An item is claimed
-// with one atomic per wave, so long extensions spread over the whole GPU.
+// The cells of one anti-diagonal depend only on the two before it, so the 16
+// lanes of an item evaluate 16 consecutive columns at once.  The item's three
+// live anti-diagonals stay in registers: lane j holds, for each of kChunks
+// chunks c, column base + 16c + j of each (a window of kWin columns starting
+// at `base`).  The value of column col - 1 comes from the neighbouring lane
+// through DPP row_shr:1, whose first lane keeps the DPP "old" operand, lane 15
+// of the chunk below rotated in with row_ror:1.  The window moves up a chunk
+// when the oldest anti-diagonal's range has left its first chunk; an item
+// whose band outgrows the window is handed on to the global-ring kernel
+// through the overflow list.  The anti-diagonal's best score is a row
+// maximum (row_ror 1, 2, 4, 8) and the band update two ballots (the first
+// column from the left and the last from the right that stay live).  Query
+// residues sit in registers per column; database residues are loaded two
+// anti-diagonals ahead and turned into substitution scores one ahead, so no
+// step waits on memory.  A row that finishes its item takes the next one
+// from a range its wave claims with one queue atomic.
 #include <limits.h>
 
 #include "pa_internal.h"
@@ -23,8 +30,10 @@ namespace pa {
 namespace {
 
 constexpr int32_t kUndef = INT_MIN / 4;
-constexpr int kCap = 1024;      // columns of one anti-diagonal ring
-constexpr int kResCap = 4096;   // residues of a segment staged in LDS
+constexpr int kW = 16;                // lanes per item
+constexpr int kChunks = 8;            // chunks of kW columns per lane
+constexpr int kWin = kW * kChunks;    // columns of an item's window
+constexpr uint32_t kClaim = 4;        // items claimed per queue atomic
 
 struct WaveArgs {
     const XdTask *tasks;
@@ -37,14 +46,30 @@ struct WaveArgs {
     uint32_t *ovf_list, *ovf_count;
 };
 
-__device__ inline int32_t wave_max(int32_t v)
+// lane j of each 16-lane row reads lane j - K (mod 16) of its row
+template <int K> __device__ inline int32_t ror(int32_t v)
 {
-#pragma unroll
-    for (int m = 1; m < 64; m <<= 1) {
-        const int32_t o = __shfl_xor(v, m);
-        v = o > v ? o : v;
-    }
-    return v;
+    return __builtin_amdgcn_update_dpp(v, v, 0x120 + K, 0xf, 0xf, false);
+}
+
+// Column col - 1 of the lanes' columns of a chunk: lanes 1..15 take lane
+// j - 1 of `cur`, lane 0 keeps the DPP "old" operand, lane 15 of `below`.
+__device__ inline int32_t left_of(int32_t cur, int32_t below)
+{
+    return __builtin_amdgcn_update_dpp(ror<1>(below), cur, 0x111, 0xf, 0xf,
+                                       false);
+}
+
+__device__ inline int32_t row_max(int32_t v)
+{
+    int32_t o = ror<1>(v);
+    v = o > v ? o : v;
+    o = ror<2>(v);
+    v = o > v ? o : v;
+    o = ror<4>(v);
+    v = o > v ? o : v;
+    o = ror<8>(v);
+    return o > v ? o : v;
 }
 
 // Writes the half of XdOut that item k (direction k & 1) owns.
@@ -71,220 +96,334 @@ __device__ inline void wave_write(const WaveArgs &a, uint32_t k, int32_t score,
     }
 }
 
-__global__ __launch_bounds__(64) void k_xdrop_wave(WaveArgs a)
+enum : int { kIdle = 0, kRun = 1, kDone = 2 };
+
+__global__ __launch_bounds__(64) void k_xdrop_band(WaveArgs a)
 {
     __shared__ int8_t mat[32 * 32];
-    __shared__ int32_t ring[3][kCap];
-    __shared__ uint8_t qres[kResCap], dres[kResCap];
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x, j = lane & (kW - 1), g = lane / kW;
     for (int i = lane; i < 32 * 32; i += 64)
         mat[i] = a.mat[i];
+    __syncthreads();
     const uint32_t n = *a.count;
     const int gap = a.gap, X = a.X;
-    constexpr int M = kCap - 1;
-    for (;;) {
-        // every lane takes part in the claim (lanes other than 0 add 0)
-        uint32_t idx = atomicAdd(a.queue, lane == 0 ? 1u : 0u);
-        idx = __builtin_amdgcn_readfirstlane(idx);
-        if (idx >= n)
-            break;
-        const uint32_t k = glb(a.list)[idx];
-        const XdTask T = a.tasks[k >> 1];
-        const uint8_t PA_GLOBAL *h = glb_addr<const uint8_t>(T.h_off);
-        const uint8_t PA_GLOBAL *v = glb_addr<const uint8_t>(T.v_off);
-        const int beg_h = (int)T.beg_h, beg_v = (int)T.beg_v;
-        const bool right = (k & 1) != 0;
-        // segments as in k_xdrop_pl: column c of the query is qb[qstep * c],
-        // row r of the database db[dstep * r], both 1-based
-        int cols, rows, qstep, dstep;
-        const uint8_t PA_GLOBAL *qb, *db;
-        if (!right) {
-            cols = beg_v + 1;
-            rows = beg_h + 1;
-            qb = v + (cols - 1);
-            qstep = -1;
-            db = h + (rows - 1);
-            dstep = -1;
-        } else {
-            cols = (int)T.nv - beg_v - a.seed_len + 1;
-            rows = (int)T.nh - beg_h - a.seed_len + 1;
-            qb = v + beg_v + a.seed_len - 1;
-            qstep = 1;
-            db = h + beg_h + a.seed_len - 1;
-            dstep = 1;
-        }
-        // the per-lane kernel has finished every item that reaches here
-        // with a non-trivial band, so both segments are non-empty
-        const bool staged = cols - 1 <= kResCap && rows - 1 <= kResCap;
-        __syncthreads();   // the previous item's last reads are done
-        if (staged) {
-            for (int c = 1 + lane; c < cols; c += 64)
-                qres[c - 1] = qb[qstep * c];
-            for (int r = 1 + lane; r < rows; r += 64)
-                dres[r - 1] = db[dstep * r];
-        }
-        // item_init: anti-diagonals 0 and 1
-        int r1 = 0, r2 = 1, r3 = 2, l1 = 0, l2 = 1, l3 = 2;
-        int off1 = 0, off2 = 0, off3 = 0;
-        int d = 1, minCol = 1, maxCol = 2, best = 0;
-        uint32_t nc = 0;
-        if (lane == 0) {
-            ring[r2][0] = 0;
-            ring[r3][0] = -gap > X ? kUndef : gap;
-            ring[r3][1] = -gap > X ? kUndef : gap;
-        }
-        bool overflow = false;
-        for (;;) {
-            __syncthreads();
-            // one anti-diagonal (item_step)
-            ++d;
-            const int t = r1;
-            r1 = r2;
-            r2 = r3;
-            r3 = t;
-            l1 = l2;
-            l2 = l3;
-            off1 = off2;
-            off2 = off3;
-            off3 = minCol - 1;
-            l3 = maxCol + 1 - off3;
-            if (l3 > kCap) {
-                overflow = true;
-                break;
+
+    // Item of this lane's row; the same in all 16 lanes of the row.
+    int state = kIdle;
+    uint32_t k = 0;
+    const uint8_t PA_GLOBAL *qb = nullptr;
+    const uint8_t PA_GLOBAL *db = nullptr;
+    int qstep = 1, dstep = 1, cols = 2, rows = 2;
+    int d = 0, minCol = 0, maxCol = 0, best = 0, base = 0;
+    int off1 = 0, off2 = 0, off3 = 0, l1 = 0, l2 = 0, l3 = 0;
+    uint32_t nc = 0;
+    // this lane's column base + kW*c + j of anti-diagonals d-2, d-1, d
+    int32_t A1[kChunks], A2[kChunks], A3[kChunks];
+    int q[kChunks];             // query residue of the column
+    int32_t sc[kChunks];        // substitution score of the next step's cell
+    int B0[kChunks], B1[kChunks];   // database residues, loaded ahead
+#pragma unroll
+    for (int c = 0; c < kChunks; ++c) {
+        A1[c] = A2[c] = A3[c] = kUndef;
+        q[c] = 0;
+        sc[c] = 0;
+        B0[c] = B1[c] = 0;
+    }
+    uint32_t cbase = 0, cend = 0;   // claimed, not yet handed out
+
+    auto qres = [&](int col) {   // query residue of column col (clamped)
+        col = col < 1 ? 1 : (col > cols - 1 ? cols - 1 : col);
+        return (int)qb[qstep * col];
+    };
+    auto dres = [&](int row) {   // database residue of row (clamped)
+        row = row < 1 ? 1 : (row > rows - 1 ? rows - 1 : row);
+        return (int)db[dstep * row];
+    };
+    // group value of column x of anti-diagonal array A (x in the window)
+    auto at = [&](const int32_t *A, int x) {
+        int32_t v = INT_MIN;
+#pragma unroll
+        for (int c = 0; c < kChunks; ++c)
+            v = base + kW * c + j == x ? A[c] : v;
+        return row_max(v);
+    };
+
+    // One half of the loop: claims for idle rows, then one anti-diagonal of
+    // every running item.  LOAD receives the database residues of the step
+    // after next, LOOK holds those of the next step (loaded a half before).
+    auto half = [&](int *LOAD, int *LOOK) {
+        // ---- claims (wave-uniform block; every lane runs it) -----------
+        uint64_t need = __ballot(state == kIdle && j == 0);
+        while (need) {
+            if (cbase >= cend) {
+                uint32_t b = atomicAdd(a.queue, lane == 0 ? kClaim : 0u);
+                b = __builtin_amdgcn_readfirstlane(b);
+                cbase = b;
+                cend = b + kClaim;
             }
-            const int maxCol0 = maxCol;
-            if (lane == 0) {
-                ring[r3][off3 & M] = kUndef;
-                ring[r3][maxCol0 & M] = kUndef;
-                if ((long)d * gap > (long)best - X) {
-                    if (off3 == 0)
-                        ring[r3][0] = d * gap;
-                    if (d - maxCol0 == 0)
-                        ring[r3][maxCol0 & M] = d * gap;
-                }
-            }
-            int32_t adbest = d * gap;
-            const int32_t lim = best - X;
-            for (int base = minCol; base < maxCol0; base += 64) {
-                const int col = base + lane;
-                if (col < maxCol0) {
-                    const int32_t u = ring[r2][col & M];
-                    const int32_t l = ring[r2][(col - 1) & M];
-                    int32_t tmp = (l > u ? l : u) + gap;
-                    const int row = d - col;
-                    const int qr = staged ? (int)qres[col - 1]
-                                          : (int)qb[qstep * col];
-                    const int dr = staged ? (int)dres[row - 1]
-                                          : (int)db[dstep * row];
-                    const int32_t dg =
-                        ring[r1][(col - 1) & M] + mat[qr * 32 + dr];
-                    tmp = dg > tmp ? dg : tmp;
-                    if (tmp < lim) {
-                        ring[r3][col & M] = kUndef;
+            const uint32_t cnt = (uint32_t)__popcll(need);
+            const uint32_t take = min(cnt, cend - cbase);
+            // rank of this row among the rows asking
+            const uint64_t mine = 1ull << (g * kW);
+            const uint32_t rank = (uint32_t)__popcll(need & (mine - 1ull));
+            const bool asks = (need & mine) != 0;
+            if (asks && rank < take) {
+                const uint32_t idx = cbase + rank;
+                if (idx >= n) {
+                    state = kDone;
+                } else {
+                    k = glb(a.list)[idx];
+                    const XdTask T = a.tasks[k >> 1];
+                    const uint8_t PA_GLOBAL *h = glb_addr<const uint8_t>(T.h_off);
+                    const uint8_t PA_GLOBAL *v = glb_addr<const uint8_t>(T.v_off);
+                    const int beg_h = (int)T.beg_h, beg_v = (int)T.beg_v;
+                    if ((k & 1) == 0) {
+                        cols = beg_v + 1;
+                        rows = beg_h + 1;
+                        qb = v + (cols - 1);
+                        qstep = -1;
+                        db = h + (rows - 1);
+                        dstep = -1;
                     } else {
-                        ring[r3][col & M] = tmp;
-                        adbest = tmp > adbest ? tmp : adbest;
+                        cols = (int)T.nv - beg_v - a.seed_len + 1;
+                        rows = (int)T.nh - beg_h - a.seed_len + 1;
+                        qb = v + beg_v + a.seed_len - 1;
+                        qstep = 1;
+                        db = h + beg_h + a.seed_len - 1;
+                        dstep = 1;
                     }
+                    // item_init: anti-diagonals 0 (A2) and 1 (A3); the
+                    // step rotates them into A1 and A2
+                    d = 1;
+                    minCol = 1;
+                    maxCol = 2;
+                    best = 0;
+                    base = 0;
+                    off1 = off2 = off3 = 0;
+                    l1 = 0;
+                    l2 = 1;
+                    l3 = 2;
+                    nc = 0;
+                    const int32_t g1 = -gap > X ? kUndef : gap;
+#pragma unroll
+                    for (int c = 0; c < kChunks; ++c) {
+                        const int col = kW * c + j;
+                        A2[c] = col == 0 ? 0 : kUndef;
+                        A3[c] = col <= 1 ? g1 : kUndef;
+                        q[c] = qres(col);
+                        // scores of step 2 (rows 2 - col) and residues of
+                        // step 3 (rows 3 - col)
+                        sc[c] = mat[q[c] * 32 + dres(2 - col)];
+                        LOOK[c] = dres(3 - col);
+                    }
+                    state = kRun;
                 }
             }
-            nc += (uint32_t)(maxCol0 - minCol);
-            adbest = wave_max(adbest);
-            best = adbest > best ? adbest : best;
-            __syncthreads();
-            // new minCol: the first column c >= minCol that does not satisfy
-            // (c in A3's range, A3[c] undefined, c-1 in A2's range, A2[c-1]
-            // undefined); maxCol0 + 1 when all of [minCol, maxCol0] do
-            int nmin = maxCol0 + 1;
-            for (int base = minCol; base <= maxCol0; base += 64) {
-                const int c = base + lane;
-                const bool dead = c - off3 < l3 &&
-                                  ring[r3][c & M] == kUndef &&
-                                  c - off2 - 1 < l2 &&
-                                  ring[r2][(c - 1) & M] == kUndef;
-                const uint64_t live = __ballot(c <= maxCol0 && !dead);
-                if (live) {
-                    nmin = base + __ffsll((unsigned long long)live) - 1;
-                    break;
-                }
-            }
-            // new maxCol: one past the last column x in [off3, maxCol0) with
-            // A3[x] or A2[x] defined (off3 when none), plus one
-            int nmax = off3;
-            for (int top = maxCol0 - 1; top >= off3; top -= 64) {
-                const int x = top - lane;
-                const bool alive = x >= off3 &&
-                                   !(ring[r3][x & M] == kUndef &&
-                                     ring[r2][x & M] == kUndef);
-                const uint64_t live = __ballot(alive);
-                if (live) {
-                    nmax = top - (__ffsll((unsigned long long)live) - 1) + 1;
-                    break;
-                }
-            }
-            ++nmax;
-            if (nmin < d + 2 - rows)
-                nmin = d + 2 - rows;
-            if (nmax > cols)
-                nmax = cols;
-            minCol = nmin;
-            maxCol = nmax;
-            if (minCol >= maxCol)
-                break;
+            cbase += take;
+            for (uint32_t t = 0; t < take; ++t)
+                need &= need - 1;
         }
-        if (overflow) {
-            if (lane == 0) {
+
+        // ---- one anti-diagonal of each running item ---------------------
+        // Every lane runs this part (the DPP moves, ballots and row maxima
+        // need whole rows, and the emulation of the tests needs the whole
+        // wave): rows without an item compute on stale registers, which the
+        // next claim resets, and only memory accesses and state changes are
+        // guarded by `run`.
+        bool run = state == kRun;
+        ++d;
+#pragma unroll
+        for (int c = 0; c < kChunks; ++c) {
+            A1[c] = A2[c];
+            A2[c] = A3[c];
+        }
+        l1 = l2;
+        l2 = l3;
+        off1 = off2;
+        off2 = off3;
+        off3 = minCol - 1;
+        l3 = maxCol + 1 - off3;
+        const int maxCol0 = maxCol;
+        if (run && maxCol0 - base >= kWin) {
+            // the band left the window: restart in the global-ring kernel
+            if (j == 0) {
                 const uint32_t pos = atomicAdd(a.ovf_count, 1u);
                 a.ovf_list[pos] = k;
                 wave_write(a, k, 0, 0, 0, false, 0, 1);
             }
-            continue;
+            state = kIdle;
+            run = false;
         }
-        __syncthreads();
-        // item_end: the "longest extension" end point
-        int lcol = l3 + off3 - 2;
-        int lrow = d - lcol;
-        int32_t ls = ring[r3][lcol & M];
-        if (ls == kUndef) {
-            if (ring[r2][(l2 + off2 - 2) & M] != kUndef) {
-                lcol = l2 + off2 - 2;
-                lrow = d - 1 - lcol;
-                ls = ring[r2][lcol & M];
-            } else if (l2 > 2 && ring[r2][(l2 + off2 - 3) & M] != kUndef) {
-                lcol = l2 + off2 - 3;
-                lrow = d - 1 - lcol;
-                ls = ring[r2][lcol & M];
-            }
-        }
-        if (ls == kUndef) {
-            // the first maximum above kUndef of anti-diagonal d - 2: per
-            // lane over its columns, then the largest value and, among
-            // equal values, the smallest column
-            int32_t bv = kUndef;
-            int bi = INT_MAX;
-            for (int i = lane; i < l1; i += 64) {
-                const int32_t x = ring[r1][(off1 + i) & M];
-                if (x > bv) {
-                    bv = x;
-                    bi = i;
-                }
-            }
+        const bool cond = (long)d * gap > (long)best - X;
+        const int32_t lim = best - X;
+        const int32_t dgap = d * gap;
+        int32_t adbest = dgap;
+        int32_t Lm[kChunks], U[kChunks];
 #pragma unroll
-            for (int m = 1; m < 64; m <<= 1) {
-                const int32_t ov = __shfl_xor(bv, m);
-                const int oi = __shfl_xor(bi, m);
-                if (ov > bv || (ov == bv && oi < bi)) {
-                    bv = ov;
-                    bi = oi;
+        for (int c = 0; c < kChunks; ++c) {
+            const int col = base + kW * c + j;
+            const int32_t u = A2[c];
+            const int32_t l = left_of(u, c ? A2[c - 1] : kUndef);
+            const int32_t dl = left_of(A1[c], c ? A1[c - 1] : kUndef);
+            int32_t tmp = (l > u ? l : u) + gap;
+            const int32_t dg = dl + sc[c];
+            tmp = dg > tmp ? dg : tmp;
+            const bool inb = col >= minCol && col < maxCol0;
+            const int32_t val = tmp < lim ? kUndef : tmp;
+            const bool edge =
+                cond && ((col == 0 && off3 == 0) ||
+                         (col == maxCol0 && d == maxCol0));
+            A3[c] = inb ? val : (edge ? dgap : kUndef);
+            adbest = inb && val > adbest ? val : adbest;
+            Lm[c] = l;
+            U[c] = u;
+        }
+        nc += (uint32_t)(maxCol0 - minCol);
+        adbest = row_max(adbest);
+        best = adbest > best ? adbest : best;
+        // new minCol: the first column c >= minCol that does not satisfy
+        // (A3[c] undefined, c - 1 in A2's range, A2[c-1] undefined);
+        // maxCol0 + 1 when all of [minCol, maxCol0] do.  New maxCol: one
+        // past the last column x in [off3, maxCol0) with A3[x] or A2[x]
+        // defined (off3 when none), plus one.
+        int nmin = maxCol0 + 1, nmax = off3;
+        bool fmin = false, fmax = false;
+        const int sh = g * kW;
+#pragma unroll
+        for (int c = 0; c < kChunks; ++c) {
+            const int col = base + kW * c + j;
+            const bool dead = A3[c] == kUndef && col - off2 - 1 < l2 &&
+                              Lm[c] == kUndef;
+            const uint32_t m =
+                (uint32_t)(__ballot(col >= minCol && col <= maxCol0 && !dead) >>
+                           sh) & 0xffffu;
+            if (!fmin && m) {
+                nmin = base + kW * c + __builtin_ctz(m);
+                fmin = true;
+            }
+        }
+#pragma unroll
+        for (int c = kChunks - 1; c >= 0; --c) {
+            const int col = base + kW * c + j;
+            const bool alive = col >= off3 && col < maxCol0 &&
+                               !(A3[c] == kUndef && U[c] == kUndef);
+            const uint32_t m =
+                (uint32_t)(__ballot(alive) >> sh) & 0xffffu;
+            if (!fmax && m) {
+                nmax = base + kW * c + (31 - __builtin_clz(m)) + 1;
+                fmax = true;
+            }
+        }
+        ++nmax;
+        if (nmin < d + 2 - rows)
+            nmin = d + 2 - rows;
+        if (nmax > cols)
+            nmax = cols;
+        minCol = nmin;
+        maxCol = nmax;
+        const bool ended = run && minCol >= maxCol;
+        if (__ballot(ended)) {
+            // item_end: the "longest extension" end point
+            int lcol = l3 + off3 - 2;
+            int lrow = d - lcol;
+            int32_t ls = at(A3, lcol);
+            const int32_t v2 = at(A2, l2 + off2 - 2);
+            const int32_t v3 = at(A2, l2 + off2 - 3);
+            if (ls == kUndef) {
+                if (v2 != kUndef) {
+                    lcol = l2 + off2 - 2;
+                    lrow = d - 1 - lcol;
+                    ls = v2;
+                } else if (l2 > 2 && v3 != kUndef) {
+                    lcol = l2 + off2 - 3;
+                    lrow = d - 1 - lcol;
+                    ls = v3;
                 }
             }
-            if (bv > kUndef) {
+            // the first maximum above kUndef of anti-diagonal d - 2: the
+            // largest value, and among equal values the smallest column
+            int32_t bv = kUndef;
+            int bc = INT_MAX;
+#pragma unroll
+            for (int c = 0; c < kChunks; ++c) {
+                const int col = base + kW * c + j;
+                if (col >= off1 && col < off1 + l1 && A1[c] > bv) {
+                    bv = A1[c];
+                    bc = col;
+                }
+            }
+            // row reduction of (value, column)
+            int32_t ov = ror<1>(bv), oc = ror<1>(bc);
+            if (ov > bv || (ov == bv && oc < bc)) {
+                bv = ov;
+                bc = oc;
+            }
+            ov = ror<2>(bv);
+            oc = ror<2>(bc);
+            if (ov > bv || (ov == bv && oc < bc)) {
+                bv = ov;
+                bc = oc;
+            }
+            ov = ror<4>(bv);
+            oc = ror<4>(bc);
+            if (ov > bv || (ov == bv && oc < bc)) {
+                bv = ov;
+                bc = oc;
+            }
+            ov = ror<8>(bv);
+            oc = ror<8>(bc);
+            if (ov > bv || (ov == bv && oc < bc)) {
+                bv = ov;
+                bc = oc;
+            }
+            if (ls == kUndef && bv > kUndef) {
                 ls = bv;
-                lcol = bi + off1;
+                lcol = bc;
                 lrow = d - 2 - lcol;
             }
+            if (ended && j == 0)
+                wave_write(a, k, ls, lcol, lrow, ls != kUndef, nc, 0);
         }
-        if (lane == 0)
-            wave_write(a, k, ls, lcol, lrow, ls != kUndef, nc, 0);
+        if (ended) {
+            state = kIdle;
+            run = false;
+        }
+        // move the window up while its first chunk is below the oldest
+        // range the next steps read (A1 of the next step starts at off2)
+        while (run && base + kW <= off2) {
+#pragma unroll
+            for (int c = 0; c < kChunks - 1; ++c) {
+                A1[c] = A1[c + 1];
+                A2[c] = A2[c + 1];
+                A3[c] = A3[c + 1];
+                q[c] = q[c + 1];
+                LOOK[c] = LOOK[c + 1];
+            }
+            base += kW;
+            const int col = base + kW * (kChunks - 1) + j;
+            A1[kChunks - 1] = A2[kChunks - 1] = A3[kChunks - 1] = kUndef;
+            q[kChunks - 1] = qres(col);
+            LOOK[kChunks - 1] = dres(d + 1 - col);
+        }
+        // scores of the next step from the residues loaded a half ago, and
+        // the residues of the step after
+        if (run) {
+#pragma unroll
+            for (int c = 0; c < kChunks; ++c) {
+                const int col = base + kW * c + j;
+                sc[c] = mat[q[c] * 32 + LOOK[c]];
+                LOAD[c] = dres(d + 2 - col);
+            }
+        }
+    };
+
+    for (;;) {
+        half(B0, B1);
+        if (!__ballot(state != kDone))
+            break;
+        half(B1, B0);
+        if (!__ballot(state != kDone))
+            break;
     }
 }
 
@@ -298,7 +437,7 @@ hipError_t launch_xdrop_wave(const XdTask *tasks, const uint32_t *list,
 {
     WaveArgs a{tasks, list, count, queue, mat, gap, xdrop, seed_len,
                out, ovf_list, ovf_count};
-    hipLaunchKernelGGL(k_xdrop_wave, dim3(grid), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(k_xdrop_band, dim3(grid), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 
@@ -307,7 +446,7 @@ int xdrop_wave_occupancy()
     static int cache = 0;
     if (!cache) {
         int n = 0;
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_xdrop_wave, 64, 0);
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_xdrop_band, 64, 0);
         cache = n > 0 ? n : 1;
     }
     return cache;

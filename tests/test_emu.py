@@ -95,11 +95,14 @@ def test_emulated_sfa_under_sanitizers(driver, tmp_path, shapes):
     (0, SHAPES[1], "as_called"), (0, None, "as_documented")])
 def test_emulated_sxa_under_sanitizers(driver, tmp_path, coop, shapes, model):
     w = small_config2(n_seq=20, n_pairs=16, seed=72)
-    got, ext, cells = run(driver, tmp_path, w, w.pairs, 1, xdrop=49,
+    # the wide bands of the as-documented reading run through the 16-lane
+    # band kernel, whose emulation is slow: three pairs (12 work items)
+    pairs = w.pairs if model == "as_called" else w.pairs[:3]
+    got, ext, cells = run(driver, tmp_path, w, pairs, 1, xdrop=49,
                           coop=coop, shapes=shapes,
                           gap_model=0 if model == "as_called" else 1)
-    ref = orc.sxa_batch(w.codes, w.offsets, w.pairs["r"], w.pairs["c"],
-                        w.pairs["seed"], oracle_params(model))
+    ref = orc.sxa_batch(w.codes, w.offsets, pairs["r"], pairs["c"],
+                        pairs["seed"], oracle_params(model))
     for gf in ("score", "matches", "cols", "gap_opens"):
         assert np.array_equal(got[gf].astype(np.int64),
                               ref[gf].astype(np.int64)), gf
