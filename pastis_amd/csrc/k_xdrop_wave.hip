@@ -31,7 +31,7 @@ namespace {
 
 constexpr int32_t kUndef = INT_MIN / 4;
 constexpr int kW = 16;                // lanes per item
-constexpr int kChunks = 8;            // chunks of kW columns per lane
+constexpr int kChunks = 12;           // chunks of kW columns per lane
 constexpr int kWin = kW * kChunks;    // columns of an item's window
 constexpr uint32_t kClaim = 4;        // items claimed per queue atomic
 
@@ -70,6 +70,18 @@ __device__ inline int32_t row_max(int32_t v)
     v = o > v ? o : v;
     o = ror<8>(v);
     return o > v ? o : v;
+}
+
+// maximum over the wave of a non-negative value
+__device__ inline int wave_max(int v)
+{
+    const int r = row_max(v);
+    const int a = __builtin_amdgcn_readlane(r, 0);
+    const int b = __builtin_amdgcn_readlane(r, 16);
+    const int c = __builtin_amdgcn_readlane(r, 32);
+    const int d = __builtin_amdgcn_readlane(r, 48);
+    const int ab = a > b ? a : b, cd = c > d ? c : d;
+    return ab > cd ? ab : cd;
 }
 
 // Writes the half of XdOut that item k (direction k & 1) owns.
@@ -258,9 +270,14 @@ __global__ __launch_bounds__(64) void k_xdrop_band(WaveArgs a)
         const int32_t lim = best - X;
         const int32_t dgap = d * gap;
         int32_t adbest = dgap;
+        // the chunks any running row reads or writes this step: columns up
+        // to its maxCol0 (the others hold columns no step reads)
+        const int cw = wave_max(run ? (maxCol0 - base) / kW : 0);
         int32_t Lm[kChunks], U[kChunks];
 #pragma unroll
         for (int c = 0; c < kChunks; ++c) {
+            if (c > cw)
+                break;
             const int col = base + kW * c + j;
             const int32_t u = A2[c];
             const int32_t l = left_of(u, c ? A2[c - 1] : kUndef);
@@ -291,6 +308,8 @@ __global__ __launch_bounds__(64) void k_xdrop_band(WaveArgs a)
         const int sh = g * kW;
 #pragma unroll
         for (int c = 0; c < kChunks; ++c) {
+            if (c > cw)
+                break;
             const int col = base + kW * c + j;
             const bool dead = A3[c] == kUndef && col - off2 - 1 < l2 &&
                               Lm[c] == kUndef;
@@ -304,6 +323,8 @@ __global__ __launch_bounds__(64) void k_xdrop_band(WaveArgs a)
         }
 #pragma unroll
         for (int c = kChunks - 1; c >= 0; --c) {
+            if (c > cw)
+                continue;
             const int col = base + kW * c + j;
             const bool alive = col >= off3 && col < maxCol0 &&
                                !(A3[c] == kUndef && U[c] == kUndef);
@@ -406,12 +427,18 @@ __global__ __launch_bounds__(64) void k_xdrop_band(WaveArgs a)
             LOOK[kChunks - 1] = dres(d + 1 - col);
         }
         // scores of the next step from the residues loaded a half ago, and
-        // the residues of the step after
+        // the residues of the step after, for the chunks the next steps can
+        // reach (the band's top moves up at most one column per step)
+        const int pw = min(kChunks - 1,
+                           wave_max(run ? (maxCol - base) / kW + 1 : 0));
         if (run) {
 #pragma unroll
             for (int c = 0; c < kChunks; ++c) {
+                if (c > pw + 1)
+                    break;
                 const int col = base + kW * c + j;
-                sc[c] = mat[q[c] * 32 + LOOK[c]];
+                if (c <= pw)
+                    sc[c] = mat[q[c] * 32 + LOOK[c]];
                 LOAD[c] = dres(d + 2 - col);
             }
         }
