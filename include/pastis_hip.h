@@ -437,11 +437,13 @@ typedef struct pa_tuning {
     int32_t tb_stage;       /* 1: the traceback copies each lane run of the
                                trace it enters into LDS with wide loads;
                                0: one byte load per step */
-    int32_t xd_wave;        /* 1 (default): X-drop items whose band outgrows
-                               the per-lane kernel's rings continue with one
-                               wave per item (bands up to 1024 columns), the
-                               rest with global-memory rings; 0: straight to
-                               the global-memory rings, one lane per item */
+    int32_t xd_wave;        /* X-drop items whose band outgrows the
+                               per-lane kernel's rings continue with 16
+                               lanes per item: 1 (default) in a 128-column
+                               window, then a 256-column one; 2: the
+                               256-column window only; 0: none.  What is
+                               left runs with global-memory rings, one lane
+                               per item */
 } pa_tuning;
 
 int pa_get_tuning(struct pa_ctx *ctx, pa_tuning *out);

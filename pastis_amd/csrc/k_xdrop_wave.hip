@@ -31,9 +31,11 @@ namespace {
 
 constexpr int32_t kUndef = INT_MIN / 4;
 constexpr int kW = 16;                // lanes per item
-constexpr int kChunks = 12;           // chunks of kW columns per lane
-constexpr int kWin = kW * kChunks;    // columns of an item's window
 constexpr uint32_t kClaim = 4;        // items claimed per queue atomic
+// window of the main pass (128 columns) and of the pass that takes its
+// overflow (256 columns; few items, so its register use does not matter)
+constexpr int kBandChunks = 8;
+constexpr int kWideChunks = 16;
 
 struct WaveArgs {
     const XdTask *tasks;
@@ -110,8 +112,12 @@ __device__ inline void wave_write(const WaveArgs &a, uint32_t k, int32_t score,
 
 enum : int { kIdle = 0, kRun = 1, kDone = 2 };
 
+// NCH chunks of 16 columns per lane: a window of 16 * NCH columns.
+template <int NCH>
 __global__ __launch_bounds__(64) void k_xdrop_band(WaveArgs a)
 {
+    constexpr int kChunks = NCH;
+    constexpr int kWin = kW * kChunks;
     __shared__ int8_t mat[32 * 32];
     const int lane = threadIdx.x, j = lane & (kW - 1), g = lane / kW;
     for (int i = lane; i < 32 * 32; i += 64)
@@ -273,7 +279,14 @@ __global__ __launch_bounds__(64) void k_xdrop_band(WaveArgs a)
         // the chunks any running row reads or writes this step: columns up
         // to its maxCol0 (the others hold columns no step reads)
         const int cw = wave_max(run ? (maxCol0 - base) / kW : 0);
-        int32_t Lm[kChunks], U[kChunks];
+        // the new minCol and maxCol from ballots taken chunk by chunk:
+        // minCol = the first column c >= minCol that does not satisfy
+        // (A3[c] undefined, c - 1 in A2's range, A2[c-1] undefined), or
+        // maxCol0 + 1 when all of [minCol, maxCol0] do; maxCol = one past the
+        // last column x in [off3, maxCol0) with A3[x] or A2[x] defined (off3
+        // when none), plus one
+        int nmin = maxCol0 + 1, nmax = off3;
+        const int sh = g * kW;
 #pragma unroll
         for (int c = 0; c < kChunks; ++c) {
             if (c > cw)
@@ -290,51 +303,25 @@ __global__ __launch_bounds__(64) void k_xdrop_band(WaveArgs a)
             const bool edge =
                 cond && ((col == 0 && off3 == 0) ||
                          (col == maxCol0 && d == maxCol0));
-            A3[c] = inb ? val : (edge ? dgap : kUndef);
+            const int32_t a3 = inb ? val : (edge ? dgap : kUndef);
+            A3[c] = a3;
             adbest = inb && val > adbest ? val : adbest;
-            Lm[c] = l;
-            U[c] = u;
+            const bool dead = a3 == kUndef && col - off2 - 1 < l2 && l == kUndef;
+            const uint32_t mlo =
+                (uint32_t)(__ballot(col >= minCol && col <= maxCol0 && !dead) >>
+                           sh) & 0xffffu;
+            const uint32_t mhi =
+                (uint32_t)(__ballot(col >= off3 && col < maxCol0 &&
+                                    !(a3 == kUndef && u == kUndef)) >> sh) &
+                0xffffu;
+            const int lo = base + kW * c + __builtin_ctz(mlo | 0x10000u);
+            const int hi = base + kW * c + (31 - __builtin_clz(mhi | 1u)) + 1;
+            nmin = mlo && lo < nmin ? lo : nmin;
+            nmax = mhi && hi > nmax ? hi : nmax;
         }
         nc += (uint32_t)(maxCol0 - minCol);
         adbest = row_max(adbest);
         best = adbest > best ? adbest : best;
-        // new minCol: the first column c >= minCol that does not satisfy
-        // (A3[c] undefined, c - 1 in A2's range, A2[c-1] undefined);
-        // maxCol0 + 1 when all of [minCol, maxCol0] do.  New maxCol: one
-        // past the last column x in [off3, maxCol0) with A3[x] or A2[x]
-        // defined (off3 when none), plus one.
-        int nmin = maxCol0 + 1, nmax = off3;
-        bool fmin = false, fmax = false;
-        const int sh = g * kW;
-#pragma unroll
-        for (int c = 0; c < kChunks; ++c) {
-            if (c > cw)
-                break;
-            const int col = base + kW * c + j;
-            const bool dead = A3[c] == kUndef && col - off2 - 1 < l2 &&
-                              Lm[c] == kUndef;
-            const uint32_t m =
-                (uint32_t)(__ballot(col >= minCol && col <= maxCol0 && !dead) >>
-                           sh) & 0xffffu;
-            if (!fmin && m) {
-                nmin = base + kW * c + __builtin_ctz(m);
-                fmin = true;
-            }
-        }
-#pragma unroll
-        for (int c = kChunks - 1; c >= 0; --c) {
-            if (c > cw)
-                continue;
-            const int col = base + kW * c + j;
-            const bool alive = col >= off3 && col < maxCol0 &&
-                               !(A3[c] == kUndef && U[c] == kUndef);
-            const uint32_t m =
-                (uint32_t)(__ballot(alive) >> sh) & 0xffffu;
-            if (!fmax && m) {
-                nmax = base + kW * c + (31 - __builtin_clz(m)) + 1;
-                fmax = true;
-            }
-        }
         ++nmax;
         if (nmin < d + 2 - rows)
             nmin = d + 2 - rows;
@@ -460,23 +447,34 @@ hipError_t launch_xdrop_wave(const XdTask *tasks, const uint32_t *list,
                              const uint32_t *count, uint32_t *queue,
                              const int8_t *mat, int gap, int xdrop,
                              int seed_len, XdOut *out, uint32_t *ovf_list,
-                             uint32_t *ovf_count, int grid, hipStream_t s)
+                             uint32_t *ovf_count, int grid, int wide,
+                             hipStream_t s)
 {
     WaveArgs a{tasks, list, count, queue, mat, gap, xdrop, seed_len,
                out, ovf_list, ovf_count};
-    hipLaunchKernelGGL(k_xdrop_band, dim3(grid), dim3(64), 0, s, a);
+    if (wide)
+        hipLaunchKernelGGL(k_xdrop_band<kWideChunks>, dim3(grid), dim3(64), 0,
+                           s, a);
+    else
+        hipLaunchKernelGGL(k_xdrop_band<kBandChunks>, dim3(grid), dim3(64), 0,
+                           s, a);
     return hipGetLastError();
 }
 
-int xdrop_wave_occupancy()
+int xdrop_wave_occupancy(int wide)
 {
-    static int cache = 0;
-    if (!cache) {
+    static int cache[2] = {0, 0};
+    if (!cache[wide ? 1 : 0]) {
         int n = 0;
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_xdrop_band, 64, 0);
-        cache = n > 0 ? n : 1;
+        if (wide)
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &n, k_xdrop_band<kWideChunks>, 64, 0);
+        else
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &n, k_xdrop_band<kBandChunks>, 64, 0);
+        cache[wide ? 1 : 0] = n > 0 ? n : 1;
     }
-    return cache;
+    return cache[wide ? 1 : 0];
 }
 
 }  // namespace pa

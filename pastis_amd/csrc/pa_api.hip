@@ -83,8 +83,10 @@ enum {
     C_ERR = 29,         // validation errors
     C_XDLONG = 30,      // X-drop items left by the 16-lane kernel
     C_XDLONGQ = 31,     // their queue head
-    C_XDWOVF = 32,      // X-drop items left by the wave-per-item kernel
+    C_XDWOVF = 32,      // X-drop items left by the 128-column band kernel
     C_XDWOVFQ = 33,     // their queue head
+    C_XDW2OVF = 34,     // items left by the 256-column band kernel
+    C_XDW2OVFQ = 35,    // their queue head
     C_HIST = 36,        // kBins x kBuckets histogram of the counting sort
     C_FILL = C_HIST + kBins * kBuckets,   // its running slots
     C_N = C_FILL + kBins * kBuckets
@@ -528,23 +530,35 @@ int run_xdrop(pa_ctx *c, const pa_pair *d_pairs, uint64_t n, hipStream_t s)
             c->stats.launches++;
         }
     }
-    // items whose band outgrew the LDS rings, one wave per item; what
-    // outgrows its rings too is left in `lng` for the global-ring kernel
-    // (the per-lane kernel is done reading `lng` by then)
+    // items whose band outgrew the LDS rings: 16 lanes per item with a
+    // 128-column window, what outgrows it with a 256-column window, the
+    // rest with rings in global memory.  Each pass's overflow list is the
+    // list the pass before last consumed (`lng` is free once the per-lane
+    // kernel is done).
     uint32_t *last = ovf, *last_n = ctr + C_XDOVF, *last_q = ctr + C_XDOVFQ;
-    if (c->tune.xd_wave) {
+    // one band pass: items of `last` in, overflow to `to`
+    auto band = [&](int wide, uint32_t *to, uint32_t *to_n,
+                    uint32_t *to_q) -> int {
         Scope t(c, 1, s);
-        HIPCHK(c, launch_xdrop_wave(P<XdTask>(c->xdt), ovf, ctr + C_XDOVF,
-                                    ctr + C_XDOVFQ, P<int8_t>(c->dmat),
-                                    c->cfg.xdrop_gap, c->cfg.xdrop,
-                                    c->cfg.seed_len, P<XdOut>(c->xdo), lng,
-                                    ctr + C_XDWOVF,
-                                    c->ncu * xdrop_wave_occupancy(), s));
+        HIPCHK(c, launch_xdrop_wave(P<XdTask>(c->xdt), last, last_n, last_q,
+                                    P<int8_t>(c->dmat), c->cfg.xdrop_gap,
+                                    c->cfg.xdrop, c->cfg.seed_len,
+                                    P<XdOut>(c->xdo), to, to_n,
+                                    c->ncu * xdrop_wave_occupancy(wide), wide,
+                                    s));
         c->stats.launches++;
-        last = lng;
-        last_n = ctr + C_XDWOVF;
-        last_q = ctr + C_XDWOVFQ;
-    }
+        last = to;
+        last_n = to_n;
+        last_q = to_q;
+        return PA_OK;
+    };
+    if (c->tune.xd_wave == 1 &&
+        (rc = band(0, lng, ctr + C_XDWOVF, ctr + C_XDWOVFQ)))
+        return rc;
+    if (c->tune.xd_wave &&
+        (rc = band(1, last == ovf ? lng : ovf, ctr + C_XDW2OVF,
+                   ctr + C_XDW2OVFQ)))
+        return rc;
     uint32_t novf = 0;
     HIPCHK(c, hipMemcpyAsync(&novf, last_n, 4, hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipStreamSynchronize(s));
@@ -928,8 +942,8 @@ int pa_set_tuning(pa_ctx *c, const pa_tuning *t)
         return fail(c, PA_ERR_ARG, "xd_coop must be 0 or 1");
     if (t->tb_stage != 0 && t->tb_stage != 1)
         return fail(c, PA_ERR_ARG, "tb_stage must be 0 or 1");
-    if (t->xd_wave != 0 && t->xd_wave != 1)
-        return fail(c, PA_ERR_ARG, "xd_wave must be 0 or 1");
+    if (t->xd_wave < 0 || t->xd_wave > 2)
+        return fail(c, PA_ERR_ARG, "xd_wave must be 0, 1 or 2");
     for (int k = 0; k < 3; ++k)
         if (t->dp_edge[k] < 1 || t->dp_edge[k] > 65535 ||
             (k > 0 && t->dp_edge[k] <= t->dp_edge[k - 1]))

@@ -242,15 +242,17 @@ hipError_t launch_xdrop_coop(const XdTask *tasks, const uint32_t *list,
                              int seed_len, XdOut *out, uint32_t *ovf_list,
                              uint32_t *ovf_count, int grid, hipStream_t s);
 int xdrop_coop_occupancy();
-// X-drop with one wave per work item (k_xdrop_wave.hip), for the items whose
-// band outgrew the per-lane kernel's LDS rings; items whose band outgrows
-// its 1024-column rings are appended to ovf_list.
+// X-drop with 16 lanes per work item (k_xdrop_wave.hip), for the items whose
+// band outgrew the per-lane kernel's LDS rings: a 128-column window, or
+// 256 columns with `wide`; items whose band outgrows the window are appended
+// to ovf_list.
 hipError_t launch_xdrop_wave(const XdTask *tasks, const uint32_t *list,
                              const uint32_t *count, uint32_t *queue,
                              const int8_t *mat, int gap, int xdrop,
                              int seed_len, XdOut *out, uint32_t *ovf_list,
-                             uint32_t *ovf_count, int grid, hipStream_t s);
-int xdrop_wave_occupancy();
+                             uint32_t *ovf_count, int grid, int wide,
+                             hipStream_t s);
+int xdrop_wave_occupancy(int wide);
 
 // Workgroups of one DP kernel variant (of a kernel shape) that fit on one
 // CU (occupancy).

@@ -12,8 +12,8 @@
 //      traceback time (a shape also fixes the trace layout);
 //   4. persistent DP workgroups per CU, by DP time;
 //   5. traceback staging, by traceback time;
-//   6. X-drop items outgrowing the per-lane rings: wave per item or global
-//      rings, by X-drop time.
+//   6. X-drop items outgrowing the per-lane rings: the band kernels' window
+//      chain or global rings, by X-drop time.
 // A candidate replaces the incumbent only when it is faster by more than
 // kMinGain, so timing noise does not flip choices.  A sample can mislead on
 // heavy-tailed length mixes (a few long tasks set a kernel's time), so when
@@ -219,13 +219,19 @@ int search(Tuner &tu, bool xdrop, pa_tuning &t)
 
     // 6. wide X-drop bands
     if (xdrop) {
-        pa_tuning cand = t;
-        cand.xd_wave = 1 - t.xd_wave;
-        Times tm;
-        if ((rc = tu.measure(cand, &tm)))
-            return rc;
-        if (faster(tm.xdrop, cur.xdrop))
-            t = cand;
+        for (int wv = 0; wv < 3; ++wv) {
+            if (wv == t.xd_wave)
+                continue;
+            pa_tuning cand = t;
+            cand.xd_wave = wv;
+            Times tm;
+            if ((rc = tu.measure(cand, &tm)))
+                return rc;
+            if (faster(tm.xdrop, cur.xdrop)) {
+                t = cand;
+                cur = tm;
+            }
+        }
     }
     return PA_OK;
 }

@@ -125,6 +125,8 @@ int main(int argc, char **argv)
             t.dp_edge[k] = v[4 + k];
         t.tb_stage = v[7];
     }
+    if (const char *e = std::getenv("EMU_XDWAVE"))
+        t.xd_wave = std::atoi(e);
     if ((rc = pa_set_tuning(c, &t)))
         return fail("pa_set_tuning", rc, c);
     if ((rc = pa_load_seqs(c, PA_SIDE_BOTH, codes.data(), offs.data(), nseq,

@@ -36,7 +36,7 @@ def driver():
 
 
 def run(driver, tmp_path, w, pairs, mode, xdrop=0, coop=0,
-        gap_model=0, shapes=None):
+        gap_model=0, shapes=None, xd_wave=None):
     inp = tmp_path / "in.bin"
     outp = tmp_path / "out.bin"
     hdr = np.array([mode, -11, -2, 6, xdrop, 2, gap_model, coop], np.int32)
@@ -51,6 +51,8 @@ def run(driver, tmp_path, w, pairs, mode, xdrop=0, coop=0,
                UBSAN_OPTIONS="print_stacktrace=1")
     if shapes is not None:
         env["EMU_SHAPES"] = ",".join(str(v) for v in shapes)
+    if xd_wave is not None:
+        env["EMU_XDWAVE"] = str(xd_wave)
     r = subprocess.run([driver, str(inp), str(outp)], env=env,
                        capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stderr[-4000:]
@@ -90,17 +92,21 @@ def test_emulated_sfa_under_sanitizers(driver, tmp_path, shapes):
 
 # the as-documented gap reading (X-drop gap -2) grows bands of hundreds of
 # columns: those items leave the per-lane kernel for the wave-per-item one
-@pytest.mark.parametrize("coop,shapes,model", [
-    (0, None, "as_called"), (1, None, "as_called"),
-    (0, SHAPES[1], "as_called"), (0, None, "as_documented")])
-def test_emulated_sxa_under_sanitizers(driver, tmp_path, coop, shapes, model):
+# (xd_wave 2: the 256-column band kernel alone)
+@pytest.mark.parametrize("coop,shapes,model,xd_wave", [
+    (0, None, "as_called", None), (1, None, "as_called", None),
+    (0, SHAPES[1], "as_called", None), (0, None, "as_documented", None),
+    (0, None, "as_documented", 2)])
+def test_emulated_sxa_under_sanitizers(driver, tmp_path, coop, shapes, model,
+                                       xd_wave):
     w = small_config2(n_seq=20, n_pairs=16, seed=72)
     # the wide bands of the as-documented reading run through the 16-lane
     # band kernel, whose emulation is slow: three pairs (12 work items)
     pairs = w.pairs if model == "as_called" else w.pairs[:3]
     got, ext, cells = run(driver, tmp_path, w, pairs, 1, xdrop=49,
                           coop=coop, shapes=shapes,
-                          gap_model=0 if model == "as_called" else 1)
+                          gap_model=0 if model == "as_called" else 1,
+                          xd_wave=xd_wave)
     ref = orc.sxa_batch(w.codes, w.offsets, pairs["r"], pairs["c"],
                         pairs["seed"], oracle_params(model))
     for gf in ("score", "matches", "cols", "gap_opens"):

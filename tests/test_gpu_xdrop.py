@@ -107,7 +107,8 @@ def test_results_do_not_depend_on_tuning():
         base = None
         for ring, dp, xd, pl, wv in [(16, 0, 0, 0, 1), (32, 0, 0, 1, 1),
                                      (16, 1, 1, 1, 0), (32, 3, 2, 0, 1),
-                                     (16, 0, 0, 1, 0), (32, 0, 0, 1, 0)]:
+                                     (16, 0, 0, 1, 0), (32, 0, 0, 1, 0),
+                                     (16, 0, 0, 1, 2)]:
             al = make_aligner(model)
             al.set_tuning(xd_ring=ring, dp_wg_per_cu=dp, xd_wg_per_cu=xd,
                           xd_coop=1 - pl, xd_wave=wv)
