@@ -21,7 +21,8 @@ dominant kernel's roofline and a CPU sample):
   C5  the end-to-end candidate stream: 1,000,000 proteins (log-normal median
       280, 5 % in families of 2-50), k = 6 k-mer candidates (strict upper
       triangle, ckthr = 1) generated on the GPU, then learned filter + exact
-      X-drop + triples against exact X-drop + triples on every candidate.
+      X-drop + triples against exact X-drop + triples on every candidate;
+      the filter is trained on labels of a uniform sample of the stream.
 --configs none skips them; --workload sw / sw3 runs C1 / C3 as the
 headline instead.
 """
@@ -404,41 +405,32 @@ def measure_sw_config(args, which):
     return res
 
 
-def c5_train_filter(args, cfg):
-    """Learned filter for the candidate stream, trained on GPU-aligner
-    labels of the candidates of a separate, smaller set from the same
-    generator (seed 205); tau calibrated on a held-out tenth for 99.5 %
-    recall with a one-sided 95 % margin.  Returns (params, mean, istd,
-    tau, info)."""
-    from pastis_amd import prefilter, synth
-    from pastis_amd.aligner import Aligner
+def c5_train_filter(al, sample):
+    """Learned filter for the candidate stream, trained on GPU-aligner labels
+    of a uniform sample of the stream's own candidates; tau calibrated on a
+    held-out tenth for 99.5 % recall with a one-sided 95 % margin.  Installs
+    the model on `al` and returns (tau, info)."""
+    from pastis_amd import prefilter
     t = time.time()
-    tw = synth.config5(scale=0.2 * args.c5_scale, seed=205)
-    tal = Aligner(cfg)
-    tal.load_seqs(tw.codes, tw.offsets)
-    cand = tal.kmer_candidates(k=6, symmetric="upper", ckthr=1)
+    y = prefilter.labels_of(al.align(sample))
+    flt = prefilter.LearnedFilter(al, seed=0)
+    f = flt.features(sample)
     rng = np.random.default_rng(7)
-    if len(cand) > 2_000_000:
-        cand = cand[np.sort(rng.choice(len(cand), 2_000_000, replace=False))]
-    y = prefilter.labels_of(tal.align(cand))
-    flt = prefilter.LearnedFilter(tal, seed=0)
-    f = flt.features(cand)
-    perm = rng.permutation(len(cand))
+    perm = rng.permutation(len(sample))
     f, y = f[perm], y[perm]
     ntr = len(y) * 9 // 10
     t1 = time.time()
     losses = flt.train(f[:ntr], y[:ntr], epochs=4, batch=4096, lr=2e-3)
     t_train = time.time() - t1
     tau = flt.calibrate(f[ntr:], y[ntr:], recall=0.995, z=1.645)
-    params, mean, istd = tal.filter_get_model()
-    tal.close()
     info = {"train_pairs": int(ntr), "train_positives": int(y[:ntr].sum()),
+            "calibration_pairs": int(len(y) - ntr),
             "train_seconds": round(t_train, 3),
             "final_loss": round(float(losses[-1]), 5),
             "tau_logit": round(float(tau), 5),
             "setup_seconds": round(time.time() - t, 1)}
     log("C5 filter: %s" % info)
-    return params, mean, istd, tau, info
+    return tau, info
 
 
 def measure_c5(args):
@@ -453,10 +445,8 @@ def measure_c5(args):
     nseq = len(w.offsets) - 1
     log("C5: %d sequences (%d residues) generated in %.1f s" %
         (nseq, len(w.codes), time.time() - t))
-    params, mean, istd, tau, finfo = c5_train_filter(args, cfg)
     al = Aligner(cfg)
     al.load_seqs(w.codes, w.offsets)
-    al.filter_set_model(params, mean, istd)
     # candidate generation: column blocks, kept resident
     kmer = dict(k=6, symmetric="upper", ckthr=1)
     nblk = 4
@@ -476,6 +466,14 @@ def measure_c5(args):
         off += m
     gen_s = gen_count_s + (time.perf_counter() - t0)
     log("C5: %d candidates in %.2f s" % (ncand, gen_s))
+    # the filter learns from a uniform sample of the stream itself (at most
+    # 2,000,000 candidates, aligned exactly for their labels)
+    full = np.zeros(ncand, PAIR_DTYPE)
+    if ncand:
+        al.d2h(full, d_all)
+    rng = np.random.default_rng(11)
+    sidx = np.sort(rng.choice(ncand, min(ncand, 2_000_000), replace=False))
+    tau, finfo = c5_train_filter(al, full[sidx])
     batch = 10_000_000
     b = min(batch, max(ncand, 1))
     d_res = al.device_alloc(b * RESULT_DTYPE.itemsize)
@@ -547,13 +545,11 @@ def measure_c5(args):
     }
     pairs = None
     if not args.no_cpu and ncand:
-        pairs = np.zeros(min(ncand, 2_000_000), PAIR_DTYPE)
-        rng = np.random.default_rng(11)
-        # the CPU sample is drawn from the resident candidates
-        idx_all = np.sort(rng.choice(ncand, len(pairs), replace=False))
-        full = np.zeros(ncand, PAIR_DTYPE)
-        al.d2h(full, d_all)
-        pairs = full[idx_all]
+        # the CPU sample is drawn from the candidates
+        rng = np.random.default_rng(13)
+        pairs = full[np.sort(rng.choice(ncand, min(ncand, 2_000_000),
+                                        replace=False))]
+    del full
     for x in (d_all, d_res, d_kept, d_idx, d_tri):
         al.device_free(x)
     al.close()

@@ -293,8 +293,12 @@ __device__ inline void item_write(const XdArgs &a, uint32_t k, int32_t score,
     }
 }
 
-enum : int { kIdle = 0, kRun = 1, kExhausted = 2 };
+enum : int { kIdle = 0, kRun = 1, kExhausted = 2, kEnd = 3 };
 constexpr uint32_t kChunk = 64;   // items claimed per queue atomic
+// Lanes whose item ended wait (kEnd) until this many have, or until no lane
+// of the wave still runs: the end-point search, the result write and the
+// claims then run once for all of them instead of in most steps for one.
+constexpr int kEndBatch = 16;
 
 template <int RING>
 __global__ __launch_bounds__(RING ? 64 : 256) void k_xdrop_pl(XdArgs a)
@@ -389,16 +393,24 @@ __global__ __launch_bounds__(RING ? 64 : 256) void k_xdrop_pl(XdArgs a)
         if (state == kRun) {
             const int st = item_step<ST>(s, A, Rr, mat, gap, X);
             if (st == 1) {
+                state = kEnd;
+            } else if (st == 2) {
+                const uint32_t pos = atomicAdd(a.ovf_count, 1u);
+                a.ovf_list[pos] = k;
+                item_write(a, k, 0, 0, 0, false, 0, 1);
+                state = kIdle;
+            }
+        }
+        const uint64_t ending = __ballot(state == kEnd);
+        if (ending && (__popcll(ending) >= kEndBatch ||
+                       !__ballot(state == kRun))) {
+            if (state == kEnd) {
+                // the rings still hold the item's last anti-diagonals
                 int32_t score;
                 int col, row;
                 bool upd;
                 item_end(s, A, score, col, row, upd);
                 item_write(a, k, score, col, row, upd, s.nc, 0);
-                state = kIdle;
-            } else if (st == 2) {
-                const uint32_t pos = atomicAdd(a.ovf_count, 1u);
-                a.ovf_list[pos] = k;
-                item_write(a, k, 0, 0, 0, false, 0, 1);
                 state = kIdle;
             }
         }

@@ -148,6 +148,9 @@ __global__ __launch_bounds__(64) void k_xdrop_band(WaveArgs a)
         B0[c] = B1[c] = 0;
     }
     uint32_t cbase = 0, cend = 0;   // claimed, not yet handed out
+    // highest chunk any running row reaches in the coming step (set at the
+    // end of each step; a new item starts in chunk 0)
+    int cw_next = 0;
 
     auto qres = [&](int col) {   // query residue of column col (clamped)
         col = col < 1 ? 1 : (col > cols - 1 ? cols - 1 : col);
@@ -278,7 +281,7 @@ __global__ __launch_bounds__(64) void k_xdrop_band(WaveArgs a)
         int32_t adbest = dgap;
         // the chunks any running row reads or writes this step: columns up
         // to its maxCol0 (the others hold columns no step reads)
-        const int cw = wave_max(run ? (maxCol0 - base) / kW : 0);
+        const int cw = cw_next;
         // the new minCol and maxCol from ballots taken chunk by chunk:
         // minCol = the first column c >= minCol that does not satisfy
         // (A3[c] undefined, c - 1 in A2's range, A2[c-1] undefined), or
@@ -416,8 +419,9 @@ __global__ __launch_bounds__(64) void k_xdrop_band(WaveArgs a)
         // scores of the next step from the residues loaded a half ago, and
         // the residues of the step after, for the chunks the next steps can
         // reach (the band's top moves up at most one column per step)
-        const int pw = min(kChunks - 1,
-                           wave_max(run ? (maxCol - base) / kW + 1 : 0));
+        const int reach = wave_max(run ? (maxCol - base) / kW : 0);
+        cw_next = min(kChunks - 1, reach);
+        const int pw = min(kChunks - 1, reach + 1);
         if (run) {
 #pragma unroll
             for (int c = 0; c < kChunks; ++c) {
