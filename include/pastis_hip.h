@@ -63,6 +63,9 @@ enum pa_gap_model {
 #define PA_FLAG_PASS 0x1     /* coverage and identity predicate held */
 #define PA_FLAG_RERUN32 0x2  /* the int16 DP saturated; int32 re-run used */
 #define PA_FLAG_EMPTY 0x4    /* local score 0: empty alignment */
+#define PA_FLAG_NOTRACE 0x8  /* pa_align_graph only: the batch's trace codes
+                                needed more memory than the graph was
+                                captured for; the pair was not aligned */
 
 typedef struct pa_config {
     int32_t mode;         /* enum pa_mode */
@@ -205,6 +208,21 @@ int pa_sw_scores_dev(struct pa_ctx *ctx, const pa_pair *d_pairs, uint64_t n,
  * (src/DistPWRunner.cpp:116-131) for score-only batches of equal size. */
 int pa_sw_scores_graph(struct pa_ctx *ctx, const pa_pair *d_pairs, uint64_t n,
                        pa_score *d_out, void *stream);
+
+/* pa_align_dev (the traced operator in the context's mode: --sfa local
+ * alignment, or --sxa X-drop + global alignment, with traceback statistics
+ * and the predicate) as a replayed HIP graph, with the caching rules of
+ * pa_sw_scores_graph.  The first call with a given (d_pairs, n, d_out,
+ * stream) aligns the batch eagerly, sizes the trace store from it (twice
+ * the batch's trace codes, at most the context's trace memory) and
+ * captures the batch's kernels with the trace store split into chunks on
+ * the device; later calls replay the graph without any host
+ * synchronisation.  A replayed batch whose trace codes do not fit in the
+ * captured chunks is not aligned: its results carry PA_FLAG_NOTRACE (align
+ * it with pa_align_dev).  Replaces the per-batch launches behind
+ * pw_aln_batch's loop (src/DistPWRunner.cpp:116-131) for traced batches. */
+int pa_align_graph(struct pa_ctx *ctx, const pa_pair *d_pairs, uint64_t n,
+                   pa_result *d_out, void *stream);
 
 /* pa_sw_scores_dev without any device allocation.  No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code

@@ -17,7 +17,7 @@ PA_MODE_SW, PA_MODE_XDROP = 0, 1
 PA_SIDE_ROW, PA_SIDE_COL, PA_SIDE_BOTH = 0, 1, 2
 PA_ENC_CODES, PA_ENC_ASCII = 0, 1
 PA_GAP_AS_CALLED, PA_GAP_AS_DOCUMENTED = 0, 1
-PA_FLAG_PASS, PA_FLAG_RERUN32, PA_FLAG_EMPTY = 1, 2, 4
+PA_FLAG_PASS, PA_FLAG_RERUN32, PA_FLAG_EMPTY, PA_FLAG_NOTRACE = 1, 2, 4, 8
 
 # Every extern "C" function of include/pastis_hip.h.
 EXPORTS = [
@@ -31,7 +31,8 @@ EXPORTS = [
     "pa_filter_set_model", "pa_filter_get_model", "pa_filter_grad",
     "pa_filter_train", "pa_filter_logits", "pa_filter_score_dev",
     "pa_filter_select_dev", "pa_set_thresholds", "pa_get_tuning",
-    "pa_set_tuning", "pa_autotune", "pa_sw_scores_graph", "pa_kmer_candidates",
+    "pa_set_tuning", "pa_autotune", "pa_sw_scores_graph", "pa_align_graph",
+    "pa_kmer_candidates",
     "pa_fasta_parse", "pa_fasta_info", "pa_fasta_residues",
     "pa_fasta_offsets", "pa_fasta_records", "pa_fasta_free", "pa_load_fasta",
     "pa_triples_dev", "pa_write_mm", "pa_sw_scores_noalloc",
@@ -166,6 +167,7 @@ def lib():
     for f in ("pa_align", "pa_sw_scores"):
         getattr(L, f).argtypes = [vp, vp, u64, vp]
     for f in ("pa_align_dev", "pa_sw_scores_dev", "pa_sw_scores_graph",
+              "pa_align_graph",
               "pa_sw_scores_noalloc"):
         getattr(L, f).argtypes = [vp, vp, u64, vp, vp]
     L.pa_xdrop_extents.argtypes = [vp, vp, u64, i32, vp]

@@ -123,6 +123,11 @@ class Aligner:
         check(self._ctx, lib().pa_sw_scores_graph(self._ctx, d_pairs, n, d_out,
                                                   None), "pa_sw_scores_graph")
 
+    def align_graph(self, d_pairs, n, d_out):
+        """pa_align_graph: the traced operator as a replayed HIP graph."""
+        check(self._ctx, lib().pa_align_graph(self._ctx, d_pairs, n, d_out,
+                                              None), "pa_align_graph")
+
     def synchronize(self):
         check(self._ctx, lib().pa_synchronize(self._ctx), "pa_synchronize")
 

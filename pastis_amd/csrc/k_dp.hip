@@ -364,7 +364,9 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
         }
         uint8_t PA_GLOBAL *tr = nullptr;
         if (TRACE)
-            tr = glb(a.trace) + glb(a.trace_off)[tid];
+            tr = glb(a.trace) +
+                 (glb(a.trace_off)[tid] -
+                  (a.trace_base ? *glb(a.trace_base) : 0ull));
 
         Key bk_lo = O::key(0, 0), bk_hi = O::key(0, 0);
         int bp_lo = 0, bp_hi = 0;
@@ -811,7 +813,9 @@ __global__ __launch_bounds__(kTbThreads) void k_traceback(TbArgs a)
     const uint8_t PA_GLOBAL *qs = glb_addr<const uint8_t>(T.q_off);
     const uint8_t PA_GLOBAL *ts = glb_addr<const uint8_t>(T.t_off);
     const int tlen = (int)T.tlen;
-    const uint8_t *trg = a.trace + glb(a.trace_off)[tid];
+    const uint8_t *trg =
+        a.trace + (glb(a.trace_off)[tid] -
+                   (a.trace_base ? *glb(a.trace_base) : 0ull));
     // Position of row i-1 in the trace layout: pass p, lane l, half h, row r
     // of the half; kept up to date as the walk moves up, so a step costs a
     // few adds.  Byte offsets within one task's trace fit in 32 bits.

@@ -199,9 +199,14 @@ __device__ inline int bin_key(const DpTask &t, const DpShapes &sh, int local,
 // Pass 1: histogram of the (bin, bucket) keys: LDS counters per block, then
 // one global atomic per key that occurs in the block.
 __global__ __launch_bounds__(256) void k_bin_hist(
-    const DpTask *__restrict__ tasks, uint32_t a, uint32_t b, DpShapes sh,
-    int local, int maxs, int open, int ext, uint32_t *hist)
+    const DpTask *__restrict__ tasks, uint32_t a, uint32_t b,
+    const uint32_t *rng, DpShapes sh, int local, int maxs, int open, int ext,
+    uint32_t *hist)
 {
+    if (rng) {   // task range from the device (chunks sized on the GPU)
+        a = rng[0];
+        b = rng[1];
+    }
     __shared__ uint32_t h[kBins * kBuckets];
     for (int k = threadIdx.x; k < kBins * kBuckets; k += blockDim.x)
         h[k] = 0;
@@ -236,11 +241,15 @@ __global__ void k_bin_scan(const uint32_t *hist, uint32_t *fill,
 // block counts its keys in LDS (the LDS atomic's old value is the task's
 // rank), reserves one range per key with a global atomic, and scatters.
 __global__ __launch_bounds__(256) void k_bin(
-    const DpTask *__restrict__ tasks, uint32_t a, uint32_t b, DpShapes sh,
-    int local, int maxs, int open, int ext, uint32_t *__restrict__ lists,
-    uint32_t cap,
-    uint32_t *fill, unsigned long long *bin_cells)
+    const DpTask *__restrict__ tasks, uint32_t a, uint32_t b,
+    const uint32_t *rng, DpShapes sh, int local, int maxs, int open, int ext,
+    uint32_t *__restrict__ lists, uint32_t cap, uint32_t *fill,
+    unsigned long long *bin_cells)
 {
+    if (rng) {
+        a = rng[0];
+        b = rng[1];
+    }
     __shared__ uint32_t h[kBins * kBuckets], base[kBins * kBuckets];
     __shared__ unsigned long long cc[kBins];
     for (int k = threadIdx.x; k < kBins * kBuckets; k += blockDim.x)
@@ -286,9 +295,13 @@ __global__ void k_trace_sizes(const DpTask *__restrict__ tasks, uint64_t n,
 
 // Empty tasks (a zero-length span) never reach a DP kernel.
 __global__ void k_empty_out(const DpTask *__restrict__ tasks, uint32_t a,
-                            uint32_t b, int local, int open, int ext,
-                            DpOut *out)
+                            uint32_t b, const uint32_t *rng, int local,
+                            int open, int ext, DpOut *out)
 {
+    if (rng) {
+        a = rng[0];
+        b = rng[1];
+    }
     const uint32_t i = a + blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= b)
         return;
@@ -342,11 +355,17 @@ __global__ void k_finalize_sw(const DpTask *__restrict__ tasks,
                               const DpOut *__restrict__ dp,
                               const uint8_t *__restrict__ tflags, uint64_t n,
                               double cov_thr, int ani_thr,
-                              pa_result *__restrict__ out)
+                              const uint32_t *err, pa_result *__restrict__ out)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
+    if (err && *err) {   // the trace did not fit the captured chunks
+        pa_result r = {};
+        r.flags = PA_FLAG_NOTRACE;
+        out[i] = r;
+        return;
+    }
     const DpTask t = tasks[i];
     const DpOut o = dp[i];
     pa_result r = {};
@@ -377,11 +396,17 @@ __global__ void k_finalize_xd(const XdTask *__restrict__ xt,
                               const DpOut *__restrict__ dp,
                               const uint8_t *__restrict__ tflags, uint64_t n,
                               int seed_cnt, double cov_thr, int ani_thr,
-                              pa_result *__restrict__ out)
+                              const uint32_t *err, pa_result *__restrict__ out)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
+    if (err && *err) {   // the trace did not fit the captured chunks
+        pa_result r = {};
+        r.flags = PA_FLAG_NOTRACE;
+        out[i] = r;
+        return;
+    }
     int best = 0;
     float best_id = 0.0f;
     for (int s = 0; s < seed_cnt; ++s) {
@@ -508,16 +533,16 @@ hipError_t bin_tasks(const DpTask *tasks, uint32_t a, uint32_t b,
                      int ext, uint32_t *lists,
                      uint32_t cap, uint32_t *counts, uint32_t *hist,
                      uint32_t *fill, unsigned long long *bin_cells,
-                     hipStream_t s)
+                     hipStream_t s, const uint32_t *rng)
 {
     if (b <= a)
         return hipSuccess;
     hipLaunchKernelGGL(k_bin_hist, dim3(blocks(b - a, 256)), dim3(256), 0, s,
-                       tasks, a, b, sh, local, maxs, open, ext, hist);
+                       tasks, a, b, rng, sh, local, maxs, open, ext, hist);
     hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(64), 0, s, hist, fill,
                        counts);
     hipLaunchKernelGGL(k_bin, dim3(blocks(b - a, 256)), dim3(256), 0, s, tasks,
-                       a, b, sh, local, maxs, open, ext, lists, cap, fill,
+                       a, b, rng, sh, local, maxs, open, ext, lists, cap, fill,
                        bin_cells);
     return hipGetLastError();
 }
@@ -533,12 +558,60 @@ hipError_t trace_sizes(const DpTask *tasks, uint64_t n, const DpShapes &sh,
 }
 
 hipError_t empty_out(const DpTask *tasks, uint32_t a, uint32_t b, int local,
-                     int open, int ext, DpOut *out, hipStream_t s)
+                     int open, int ext, DpOut *out, hipStream_t s,
+                     const uint32_t *rng)
 {
     if (b <= a)
         return hipSuccess;
     hipLaunchKernelGGL(k_empty_out, dim3(blocks(b - a, 256)), dim3(256), 0, s,
-                       tasks, a, b, local, open, ext, out);
+                       tasks, a, b, rng, local, open, ext, out);
+    return hipGetLastError();
+}
+
+// Chunks of tasks [0, n) whose trace codes fit in `cap` bytes, from the
+// exclusive scan toffs (n + 1 entries): chunk k is tasks [a_k, b_k) with the
+// largest b_k such that toffs[b_k] - toffs[a_k] <= cap, and its base offset
+// toffs[a_k].  Chunks past the end are empty.  *err is set when a single
+// task needs more than `cap` or the tasks need more than nchunks chunks.
+__global__ void k_trace_chunks(const uint64_t *__restrict__ toffs, uint32_t n,
+                               uint64_t cap, int nchunks, uint32_t *ranges,
+                               uint64_t *bases, uint32_t *err)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0)
+        return;
+    uint32_t a = 0;
+    for (int k = 0; k < nchunks; ++k) {
+        uint32_t b = a;
+        if (a < n) {
+            uint32_t lo = a, hi = n;   // largest b in [a, n] that fits
+            while (lo < hi) {
+                const uint32_t mid = lo + (hi - lo + 1) / 2;
+                if (toffs[mid] - toffs[a] <= cap)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            b = lo;
+            if (b == a) {   // one task larger than the store
+                *err = 1u;
+                b = a = n;
+            }
+        }
+        ranges[2 * k] = a;
+        ranges[2 * k + 1] = b;
+        bases[k] = toffs[a];
+        a = b;
+    }
+    if (a < n)
+        *err = 1u;
+}
+
+hipError_t trace_chunks(const uint64_t *toffs, uint32_t n, uint64_t cap,
+                        int nchunks, uint32_t *ranges, uint64_t *bases,
+                        uint32_t *err, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_trace_chunks, dim3(1), dim3(64), 0, s, toffs, n, cap,
+                       nchunks, ranges, bases, err);
     return hipGetLastError();
 }
 
@@ -555,25 +628,26 @@ hipError_t mark_flags(const uint32_t *list, const uint32_t *count,
 
 hipError_t finalize_sw(const DpTask *tasks, const DpOut *dp,
                        const uint8_t *tflags, uint64_t n, double cov_thr,
-                       int ani_thr, pa_result *out, hipStream_t s)
+                       int ani_thr, pa_result *out, hipStream_t s,
+                       const uint32_t *err)
 {
     if (!n)
         return hipSuccess;
     hipLaunchKernelGGL(k_finalize_sw, dim3(blocks(n, 256)), dim3(256), 0, s,
-                       tasks, dp, tflags, n, cov_thr, ani_thr, out);
+                       tasks, dp, tflags, n, cov_thr, ani_thr, err, out);
     return hipGetLastError();
 }
 
 hipError_t finalize_xd(const XdTask *xt, const XdOut *xo, const DpOut *dp,
                        const uint8_t *tflags,
                        uint64_t n, int seed_cnt, double cov_thr, int ani_thr,
-                       pa_result *out, hipStream_t s)
+                       pa_result *out, hipStream_t s, const uint32_t *err)
 {
     if (!n)
         return hipSuccess;
     hipLaunchKernelGGL(k_finalize_xd, dim3(blocks(n, 256)), dim3(256), 0, s,
                        xt, xo, dp, tflags, n, seed_cnt, cov_thr,
-                       ani_thr, out);
+                       ani_thr, err, out);
     return hipGetLastError();
 }
 

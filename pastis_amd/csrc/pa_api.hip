@@ -95,6 +95,7 @@ enum {
 constexpr int kMaxWgPerCu = 32;     // one-wave workgroups per CU (8 per SIMD)
 constexpr int kSlotsPerWg = 4;      // task slots of a 16-lane-group wave
 constexpr int kXdWideLanes = 65536;  // lanes of the wide (global scratch) re-run
+constexpr int kXdGraphLanes = 4096;  // its lanes in a captured batch
 
 }  // namespace
 
@@ -221,6 +222,15 @@ int ensure_bnd(pa_ctx *c)
     return grow(c, c->bnd, need);
 }
 
+// Overflow flag of the device-sized trace chunks (after the ranges and
+// bases in gchunk).
+uint32_t *gchunk_err(pa_ctx *c)
+{
+    return (uint32_t *)((uint64_t *)(P<uint32_t>(c->gchunk) +
+                                     2 * c->g_chunks) +
+                        c->g_chunks);
+}
+
 // Runs the DP kernels (and the traceback when traced) on tasks [0, nt).
 int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
            hipStream_t s)
@@ -231,7 +241,35 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
     // a fill kernel, not a memset node, so that the sequence can be captured
     HIPCHK(c, fill_u32(P<uint32_t>(c->tflags), 0, (uint32_t)((nt + 3) / 4), s));
     std::vector<std::pair<uint64_t, uint64_t>> chunks;
-    if (trace) {
+    if (trace && c->g_chunks > 0) {
+        // captured batch: the trace store (g_cap bytes) is split into
+        // g_chunks chunks on the device; no host synchronisation
+        if ((rc = grow(c, c->sizes, nt * sizeof(uint64_t) + 8)))
+            return rc;
+        if ((rc = grow(c, c->toffs, nt * sizeof(uint64_t) + 8)))
+            return rc;
+        HIPCHK(c, trace_sizes(P<DpTask>(c->tasks), nt, c->shapes,
+                              P<uint64_t>(c->sizes), s));
+        // sizes[nt] = 0 so that toffs[nt] is the total (a fill kernel: no
+        // memset node in a captured graph)
+        HIPCHK(c, fill_u32((uint32_t *)(P<uint64_t>(c->sizes) + nt), 0, 2, s));
+        size_t tmp = 0;
+        HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(
+                      nullptr, tmp, P<uint64_t>(c->sizes),
+                      P<uint64_t>(c->toffs), (int)(nt + 1), s));
+        if (tmp > c->cub.cap)
+            return fail(c, PA_ERR_HIP, "scan workspace grew during capture");
+        HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(
+                      c->cub.p, tmp, P<uint64_t>(c->sizes),
+                      P<uint64_t>(c->toffs), (int)(nt + 1), s));
+        HIPCHK(c, trace_chunks(P<uint64_t>(c->toffs), (uint32_t)nt, c->g_cap,
+                               c->g_chunks, P<uint32_t>(c->gchunk),
+                               (uint64_t *)(P<uint32_t>(c->gchunk) +
+                                            2 * c->g_chunks),
+                               gchunk_err(c), s));
+        for (int k = 0; k < c->g_chunks; ++k)
+            chunks.push_back({0, nt});
+    } else if (trace) {
         if ((rc = grow(c, c->sizes, nt * sizeof(uint64_t) + 8)))
             return rc;
         if ((rc = grow(c, c->toffs, nt * sizeof(uint64_t) + 8)))
@@ -294,11 +332,20 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
     const int dmode = local ? (trace ? 1 : 0) : 2;
     // tracebacks on the second stream, overlapping later classes' DP
     const bool split = trace && !c->capturing && c->st2;
-    for (auto &ch : chunks) {
+    const bool dev_chunks = trace && c->g_chunks > 0;
+    for (size_t ci = 0; ci < chunks.size(); ++ci) {
+        const auto &ch = chunks[ci];
         const uint32_t a = (uint32_t)ch.first, b = (uint32_t)ch.second;
         uint64_t base = 0;
-        if (trace && chunks.size() > 1)
+        if (trace && !dev_chunks && chunks.size() > 1)
             base = c->h_toffs[a];
+        // device-sized chunk ci: task range and trace base from the device
+        const uint32_t *rng =
+            dev_chunks ? P<uint32_t>(c->gchunk) + 2 * ci : nullptr;
+        const uint64_t *tbase =
+            dev_chunks ? (const uint64_t *)(P<uint32_t>(c->gchunk) +
+                                            2 * c->g_chunks) + ci
+                       : nullptr;
         HIPCHK(c, fill_u32(ctr, 0, C_XDQ, s));
         HIPCHK(c, fill_u32(ctr + C_HIST, 0, C_N - C_HIST, s));
         HIPCHK(c, bin_tasks(P<DpTask>(c->tasks), a, b, c->shapes,
@@ -308,10 +355,10 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
                             ctr + C_FILL,
                             (unsigned long long *)P<uint64_t>(c->bincells) +
                                 dmode * kBins,
-                            s));
+                            s, rng));
         HIPCHK(c, empty_out(P<DpTask>(c->tasks), a, b, local ? 1 : 0,
                             c->cfg.gap_open, c->cfg.gap_ext,
-                            P<DpOut>(c->dpout), s));
+                            P<DpOut>(c->dpout), s, rng));
         DpArgs args{};
         args.codes = nullptr;  // task offsets hold absolute device addresses
         args.tasks = P<DpTask>(c->tasks);
@@ -323,6 +370,7 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
         args.out = P<DpOut>(c->dpout);
         args.trace = trace ? P<uint8_t>(c->trace) - base : nullptr;
         args.trace_off = trace ? P<uint64_t>(c->toffs) : nullptr;
+        args.trace_base = tbase;
         args.bnd = P<int32_t>(c->bnd);
         args.bnd_stride = 2u * (c->max_clen + 2);
         args.gap_open = c->cfg.gap_open;
@@ -332,6 +380,7 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
         tb.tasks = P<DpTask>(c->tasks);
         tb.trace = trace ? P<uint8_t>(c->trace) - base : nullptr;
         tb.trace_off = P<uint64_t>(c->toffs);
+        tb.trace_base = tbase;
         tb.out = P<DpOut>(c->dpout);
         tb.local = local ? 1 : 0;
         // traceback of one class's tasks: its two bin lists (bin = 2 * class
@@ -471,7 +520,8 @@ int align_sw_dev(pa_ctx *c, const pa_pair *d_pairs, uint64_t n,
         return rc;
     HIPCHK(c, finalize_sw(P<DpTask>(c->tasks), P<DpOut>(c->dpout),
                           P<uint8_t>(c->tflags), n, c->cfg.cov_thr,
-                          c->cfg.ani_thr, d_out, s));
+                          c->cfg.ani_thr, d_out, s,
+                          c->g_chunks > 0 ? gchunk_err(c) : nullptr));
     c->stats.pairs += n;
     return PA_OK;
 }
@@ -559,6 +609,20 @@ int run_xdrop(pa_ctx *c, const pa_pair *d_pairs, uint64_t n, hipStream_t s)
         (rc = band(1, last == ovf ? lng : ovf, ctr + C_XDW2OVF,
                    ctr + C_XDW2OVFQ)))
         return rc;
+    if (c->g_chunks > 0) {
+        // captured batch: no host synchronisation; the global-ring pass
+        // always runs, on the kept scratch (its grid exits when its list
+        // is empty)
+        Scope t(c, 1, s);
+        HIPCHK(c, launch_xdrop(P<XdTask>(c->xdt), last, last_n, last_q,
+                               P<int8_t>(c->dmat), c->cfg.xdrop_gap,
+                               c->cfg.xdrop, c->cfg.seed_len,
+                               P<int32_t>(c->xdscr), c->xdscr_cap,
+                               P<XdOut>(c->xdo), last == ovf ? lng : ovf,
+                               ctr + C_XDDUMMY, kXdGraphLanes / 256, 0, s));
+        c->stats.launches++;
+        return PA_OK;
+    }
     uint32_t novf = 0;
     HIPCHK(c, hipMemcpyAsync(&novf, last_n, 4, hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipStreamSynchronize(s));
@@ -617,7 +681,8 @@ int align_xd_dev(pa_ctx *c, const pa_pair *d_pairs, uint64_t n,
     HIPCHK(c, finalize_xd(P<XdTask>(c->xdt), P<XdOut>(c->xdo),
                           P<DpOut>(c->dpout), P<uint8_t>(c->tflags), n,
                           c->cfg.seed_cnt, c->cfg.cov_thr, c->cfg.ani_thr,
-                          d_out, s));
+                          d_out, s,
+                          c->g_chunks > 0 ? gchunk_err(c) : nullptr));
     c->stats.pairs += n;
     return PA_OK;
 }
@@ -914,7 +979,8 @@ void pa_destroy(pa_ctx *c)
                    &c->sizes, &c->toffs, &c->lists, &c->sat, &c->ctr,
                    &c->trace, &c->bnd, &c->xdt, &c->xdo, &c->ovf, &c->rsk,
                    &c->rkc, &c->ckc, &c->tri_flag, &c->tri_pos,
-                   &c->cub, &c->cells, &c->dmat, &c->ext, &c->bincells})
+                   &c->cub, &c->cells, &c->dmat, &c->ext, &c->bincells,
+                   &c->gchunk, &c->xdscr})
         release(*b);
     for (auto e : c->free_events)
         hipEventDestroy(e);
@@ -1135,7 +1201,8 @@ int pa_sw_scores_graph(pa_ctx *c, const pa_pair *d_pairs, uint64_t n,
     }
     pa_ctx::GraphEntry *hit = nullptr;
     for (auto &e : G)
-        if (e.pairs == d_pairs && e.n == n && e.out == d_out && e.s == s)
+        if (e.kind == 0 && e.pairs == d_pairs && e.n == n &&
+            e.out == d_out && e.s == s)
             hit = &e;
     if (!hit) {
         // every buffer the batch uses exists before the capture starts; a
@@ -1193,6 +1260,135 @@ int pa_sw_scores_graph(pa_ctx *c, const pa_pair *d_pairs, uint64_t n,
     hit->used = ++c->graph_tick;
     HIPCHK(c, hipGraphLaunch(hit->x, s));
     // least recently used first, so that eviction takes entry 0
+    std::sort(G.begin(), G.end(),
+              [](const pa_ctx::GraphEntry &a, const pa_ctx::GraphEntry &b) {
+                  return a.used < b.used;
+              });
+    c->stats.pairs += n;
+    return PA_OK;
+}
+
+int pa_align_graph(pa_ctx *c, const pa_pair *d_pairs, uint64_t n,
+                   pa_result *d_out, void *stream)
+{
+    int rc;
+    if ((rc = check_ready(c)))
+        return rc;
+    if (n == 0)
+        return PA_OK;
+    if (!d_pairs || !d_out)
+        return fail(c, PA_ERR_ARG, "null pointer");
+    if (n > 0x7fffffffull / 2)
+        return fail(c, PA_ERR_LIMIT, "batch larger than 2^30 pairs");
+    hipSetDevice(c->device);
+    hipStream_t s = pick(c, stream);
+    auto &G = c->graphs;
+    auto drop = [&](size_t i) {
+        hipGraphExecDestroy(G[i].x);
+        hipGraphDestroy(G[i].g);
+        G.erase(G.begin() + (long)i);
+    };
+    for (size_t i = 0; i < G.size();) {
+        if (G[i].gen != c->gen)
+            drop(i);
+        else
+            ++i;
+    }
+    pa_ctx::GraphEntry *hit = nullptr;
+    for (auto &e : G)
+        if (e.kind == 1 && e.pairs == d_pairs && e.n == n && e.out == d_out &&
+            e.s == s)
+            hit = &e;
+    if (!hit) {
+        // 1. the batch eagerly: results for this call, every buffer at its
+        //    size, and the batch's trace size
+        if ((rc = pa_align_dev(c, d_pairs, n, d_out, stream)))
+            return rc;
+        const uint64_t nt = c->cfg.mode == PA_MODE_SW
+                                ? n
+                                : n * (uint64_t)c->cfg.seed_cnt;
+        uint64_t total = 0;
+        HIPCHK(c, hipMemcpyAsync(&total, P<uint64_t>(c->toffs) + nt, 8,
+                                 hipMemcpyDeviceToHost, s));
+        HIPCHK(c, hipStreamSynchronize(s));
+        // 2. the captured layout: a store of twice this batch's trace
+        //    (within the context's trace memory), split into as many chunks
+        //    as this batch needs plus one
+        const uint64_t cap = std::max<uint64_t>(
+            64, std::min<uint64_t>(c->trace_cap, 2 * total + (1u << 20)));
+        const int chunks = (int)std::min<uint64_t>(
+            64, (total + cap - 1) / cap + 1);
+        if ((rc = grow(c, c->trace, cap)) ||
+            (rc = grow(c, c->gchunk, (size_t)chunks * 16 + 16)))
+            return rc;
+        if (c->cfg.mode == PA_MODE_XDROP) {
+            int rcap = 64;
+            while (rcap < (int)std::max(c->max_rlen, c->max_clen) + 4)
+                rcap <<= 1;
+            if ((rc = grow(c, c->xdscr, (size_t)kXdGraphLanes * 3 *
+                                            (size_t)rcap * sizeof(int32_t))))
+                return rc;
+            c->xdscr_cap = rcap;
+        }
+        c->g_cap = cap;
+        for (size_t i = 0; i < G.size();) {
+            if (G[i].gen != c->gen ||
+                (G.size() >= (size_t)pa_ctx::kGraphs && i == 0))
+                drop(i);
+            else
+                ++i;
+        }
+        // 3. capture
+        HIPCHK(c, hipStreamSynchronize(s));
+        HIPCHK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        c->capturing = true;
+        c->g_chunks = chunks;
+        // the chunk overflow flag starts at zero on every replay
+        rc = fill_u32(gchunk_err(c), 0, 1, s) != hipSuccess ? PA_ERR_HIP
+                                                            : PA_OK;
+        const uint64_t gen0 = c->gen;
+        if (!rc)
+            rc = c->cfg.mode == PA_MODE_SW
+                     ? align_sw_dev(c, d_pairs, n, d_out, s)
+                     : align_xd_dev(c, d_pairs, n, d_out, s);
+        c->g_chunks = 0;
+        c->capturing = false;
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(s, &g);
+        if (!rc && c->gen != gen0)
+            rc = fail(c, PA_ERR_HIP, "workspace reallocated during capture");
+        if (rc || e != hipSuccess) {
+            if (g)
+                hipGraphDestroy(g);
+            return rc ? rc : fail(c, PA_ERR_HIP, "graph capture failed");
+        }
+        pa_ctx::GraphEntry ent;
+        ent.g = g;
+        if (hipGraphInstantiate(&ent.x, g, nullptr, nullptr, 0) !=
+            hipSuccess) {
+            hipGraphDestroy(g);
+            return fail(c, PA_ERR_HIP, "graph instantiation failed");
+        }
+        ent.pairs = d_pairs;
+        ent.n = n;
+        ent.out = d_out;
+        ent.s = s;
+        ent.gen = c->gen;
+        ent.kind = 1;
+        G.push_back(ent);
+        c->stats.graph_captures++;
+        std::sort(G.begin(), G.end(),
+                  [](const pa_ctx::GraphEntry &a,
+                     const pa_ctx::GraphEntry &b) { return a.used < b.used; });
+        // this call's results came from the eager run
+        for (auto &x : G)
+            if (x.kind == 1 && x.pairs == d_pairs && x.n == n &&
+                x.out == d_out && x.s == s)
+                x.used = ++c->graph_tick;
+        return PA_OK;
+    }
+    hit->used = ++c->graph_tick;
+    HIPCHK(c, hipGraphLaunch(hit->x, s));
     std::sort(G.begin(), G.end(),
               [](const pa_ctx::GraphEntry &a, const pa_ctx::GraphEntry &b) {
                   return a.used < b.used;

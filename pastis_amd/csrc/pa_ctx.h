@@ -111,7 +111,16 @@ struct pa_ctx {
         uint64_t n = 0;
         hipStream_t s = nullptr;
         uint64_t gen = 0, used = 0;
+        int kind = 0;             // 0 pa_sw_scores_graph, 1 pa_align_graph
     };
+    // Traced batches captured by pa_align_graph: while capturing, g_chunks
+    // (> 0) trace-store chunks of g_cap bytes are sized on the device
+    // (gchunk: task ranges, base offsets, overflow flag) and the X-drop
+    // global-ring pass always runs, on the kept scratch xdscr.
+    int g_chunks = 0;
+    uint64_t g_cap = 0;
+    Buf gchunk, xdscr;
+    int xdscr_cap = 0;           // ring slots of xdscr per lane
     static constexpr int kGraphs = 8;
     std::vector<GraphEntry> graphs;
     uint64_t graph_tick = 0;

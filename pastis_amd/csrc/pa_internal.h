@@ -90,6 +90,8 @@ struct DpArgs {
     DpOut *out;
     uint8_t *trace;            // trace store (traced kernels)
     const uint64_t *trace_off; // per task byte offset into trace
+    const uint64_t *trace_base;// device: offset of the chunk's first task
+                               // (chunks sized on the GPU), or null
     int32_t *bnd;              // per-workgroup boundary rows (multi-pass)
     uint32_t bnd_stride;       // int32 entries per workgroup
     uint32_t *sat_list;        // tasks that saturated (int16 kernels)
@@ -106,6 +108,7 @@ struct TbArgs {
     const uint32_t *count;
     const uint8_t *trace;
     const uint64_t *trace_off;
+    const uint64_t *trace_base;   // as DpArgs::trace_base
     DpOut *out;
     int local;
 };
