@@ -592,6 +592,8 @@ def main():
     ap.add_argument("--set-tuning", default="",
                     help="launch shapes to install, e.g. tb_stage=1,"
                          "dp_shape=0:1:2:3 (profiling runs)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="skip timing pa_align_graph replays")
     ap.add_argument("--no-operator", action="store_true",
                     help="skip timing HipAligner::aln_batch on host tuples")
     ap.add_argument("--no-filter", action="store_true",
@@ -698,6 +700,26 @@ def main():
     ms_per_step = elapsed * 1e3 / args.steps
     out = np.zeros(n, out_dtype)
     al.d2h(out, d_out)
+    graph = None
+    if args.workload == "xdrop" and not args.no_graph:
+        # the same steps replayed from a captured HIP graph (pa_align_graph:
+        # trace store chunked on the device, no host synchronisation)
+        d_gout = al.device_alloc(n * out_dtype.itemsize)
+        al.align_graph(d_pairs, n, d_gout)     # eager run + capture
+        al.synchronize()
+        al.align_graph(d_pairs, n, d_gout)     # first replay
+        al.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            al.align_graph(d_pairs, n, d_gout)
+        al.synchronize()
+        g_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        gout = np.zeros(n, out_dtype)
+        al.d2h(gout, d_gout)
+        al.device_free(d_gout)
+        graph = {"ms_per_step": round(g_ms, 3),
+                 "results_identical_to_eager": bool(np.array_equal(gout, out))}
+        log("graph replays: %.1f ms per step" % g_ms)
 
     line = {
         "metric": METRIC, "value": round(gcups, 3), "unit": "GCUPS",
@@ -713,6 +735,7 @@ def main():
                    "seed_count": 2, "k": 6, "matrix": "BLOSUM62",
                    "tuning": tuning},
         "ms_per_step_default_tuning": round(default_ms, 3),
+        "graph_replay": graph,
         "autotune_seconds": None if tune_s is None else round(tune_s, 2),
         "pairs_per_s": round(n * args.steps / elapsed, 1),
         "cells_per_step": int(cells / args.steps),

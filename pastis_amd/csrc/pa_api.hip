@@ -331,7 +331,9 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
     uint32_t *sat = P<uint32_t>(c->sat);
     const int dmode = local ? (trace ? 1 : 0) : 2;
     // tracebacks on the second stream, overlapping later classes' DP
-    const bool split = trace && !c->capturing && c->st2;
+    // (also inside a pa_align_graph capture: the event fork and join
+    // bring the second stream into the graph)
+    const bool split = trace && c->st2 && (!c->capturing || c->g_chunks > 0);
     const bool dev_chunks = trace && c->g_chunks > 0;
     for (size_t ci = 0; ci < chunks.size(); ++ci) {
         const auto &ch = chunks[ci];
