@@ -124,9 +124,16 @@ def roofline(kstats, st, step_ms_total, tuning=None, traffic_key=None):
                           peak=PEAK_I16_TOPS if k["name"].endswith("i16")
                           else PEAK_I32_TOPS))
     if st.get("ms_xdrop", 0) > 0:
+        # the X-drop stage is timed as one: its kernels chained by
+        # overflow lists, the last ones taking only the items that outgrow
+        # the earlier ones' windows
         tn = tuning or {}
-        xk = ("k_xdrop_coop + k_xdrop_pl<%d>" % tn.get("xd_ring", 16)
-              if tn.get("xd_coop") else "k_xdrop_pl<%d>" % tn.get("xd_ring", 16))
+        chain = ["k_xdrop_coop"] if tn.get("xd_coop") else []
+        chain.append("k_xdrop_pl<%d>" % tn.get("xd_ring", 16))
+        chain += {0: [], 1: ["k_xdrop_band<8>", "k_xdrop_band<16>"],
+                  2: ["k_xdrop_band<16>"]}[tn.get("xd_wave", 1)]
+        chain.append("k_xdrop_pl<0>")
+        xk = "X-drop stage: " + " -> ".join(chain)
         cands.append(dict(kernel=xk, ms=st["ms_xdrop"],
                           launches=None, cells=st["xdrop_cells"], ops=7,
                           peak=PEAK_I32_TOPS))
