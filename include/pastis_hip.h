@@ -433,7 +433,9 @@ enum pa_dp_shape {
     PA_SHAPE_L16R4 = 0,     /* 128 rows per pass */
     PA_SHAPE_L16R8 = 1,     /* 256 */
     PA_SHAPE_L64R4 = 2,     /* 512 */
-    PA_SHAPE_L64R8 = 3      /* 1024 */
+    PA_SHAPE_L64R8 = 3,     /* 1024 */
+    PA_SHAPE_L32R8 = 4,     /* 512 */
+    PA_SHAPE_L8R8 = 5       /* 128 */
 };
 
 typedef struct pa_tuning {

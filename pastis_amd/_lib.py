@@ -87,7 +87,8 @@ class PaKernelStat(C.Structure):
                 ("cells", C.c_uint64), ("ms", C.c_double)]
 
 
-DP_SHAPES = {"l16r4": 0, "l16r8": 1, "l64r4": 2, "l64r8": 3}
+DP_SHAPES = {"l16r4": 0, "l16r8": 1, "l64r4": 2, "l64r8": 3, "l32r8": 4,
+             "l8r8": 5}
 TUNE_TARGETS = {"sw_scores": 0, "align": 1}
 
 

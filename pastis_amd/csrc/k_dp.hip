@@ -1,14 +1,15 @@
 // k_dp.hip -- wavefront dynamic-programming kernels for gfx950.
 //
-// One DP task per wavefront.  The row sequence (SeqAn's vertical sequence,
-// seqsr) is split over the 64 lanes in strips of 2R rows: the upper R rows
-// ("lo") and the lower R rows ("hi") of a lane share one register per row,
-// packed int16 lo/hi, and the hi half runs one column behind the lo half so
-// both halves advance with the same instruction.  Lane l works on column
-// t-2l (lo) and t-2l-1 (hi) at step t; the H and F values of its last row go
-// to lane l+1 through a DPP wave_shr:1 move.  Columns (the horizontal
-// sequence, seqsc) stream through; a pass covers 128R rows and longer row
-// sequences take several passes linked by a boundary row in global memory.
+// One DP task per L lanes of a wavefront (L = 64, 32, 16 or 8).  The row
+// sequence (SeqAn's vertical sequence, seqsr) is split over the task's
+// lanes in strips of 2R rows: the upper R rows ("lo") and the lower R rows
+// ("hi") of a lane share one register per row, packed int16 lo/hi, and the
+// hi half runs one column behind the lo half so both halves advance with
+// the same instruction.  Lane l works on column t-2l (lo) and t-2l-1 (hi) at
+// step t; the H and F values of its last row go to lane l+1 through a DPP
+// shift.  Columns (the horizontal sequence, seqsc) stream through; a pass
+// covers 2RL rows and longer row sequences take several passes linked by a
+// boundary row in global memory.
 //
 // The substitution scores come from a query profile in LDS: for each residue
 // code c and each row, score(row residue, c) as an int8.
@@ -43,6 +44,23 @@ constexpr uint32_t kSelB1 = 0x0a050801u;
 constexpr uint32_t kSelB3 = 0x0b070903u;
 // bytes 0 and 2 of the low source, then bytes 0 and 2 of the high source
 constexpr uint32_t kSelEven = 0x06040200u;
+
+// Value of lane - 1 of the same task (L lanes per task, tasks on aligned
+// lane groups); the first lane of a task (fl) takes `old`.  DPP wave_shr:1
+// for 64- and 32-lane tasks, row_shr:1 (within 16-lane rows) for 16- and
+// 8-lane tasks; a lane whose DPP source lane has no value keeps the "old"
+// operand, which covers the first lane of a 64- or 16-lane task, and a
+// select covers the first lane of a 32- or 8-lane task, whose DPP source is
+// the last lane of the task before it.
+template <int L> __device__ inline int lane_up(int old, int src, bool fl)
+{
+    constexpr int CTRL = L >= 32 ? 0x138 : 0x111;
+    const int v = __builtin_amdgcn_update_dpp(old, src, CTRL, 0xf, 0xf, false);
+    if constexpr (L == 32 || L == 8)
+        return fl ? old : v;
+    else
+        return v;
+}
 
 template <bool I16> struct Ops;
 
@@ -101,15 +119,36 @@ template <> struct Ops<true> {
             s[4 * w + 3] = v(__builtin_amdgcn_perm(h, l, kSelB3));
         }
     }
-    // new row-above input: lo <- hi of lane-1, hi <- own lo (CTRL: DPP
-    // wave_shr:1 for 64-lane tasks, row_shr:1 for 16-lane tasks).  The
-    // first lane of a task has no source lane and keeps the DPP "old"
-    // operand, which carries `first` (the boundary value) in its high half.
-    template <int CTRL> __device__ static inline V shuffle_up(V out, int first)
+    // 4-bit profile words of 8 rows (lw: lo strip, hw: hi strip; row r in
+    // byte r % 4, high nibble for r >= 4) as int16 pairs n * MUL + base
+    template <int MUL>
+    __device__ static inline void unpack_nib(uint32_t lw, uint32_t hw,
+                                             int base, V *s)
+    {
+        typedef unsigned short u2v __attribute__((ext_vector_type(2)));
+        const V B = splat(base);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            // [byte k of lw, 0, byte k of hw, 0]
+            const uint32_t sel = (uint32_t)k | 0x0c00u |
+                                 ((uint32_t)(4 + k) << 16) | 0x0c000000u;
+            const uint32_t p = __builtin_amdgcn_perm(hw, lw, sel);
+            const V n0 = v(p & 0x000f000fu);
+            const V n1 = __builtin_bit_cast(
+                V, __builtin_bit_cast(u2v, p) >> (unsigned short)4);
+            s[k] = MUL == 1 ? add(n0, B) : n0 * splat(MUL) + B;
+            s[k + 4] = MUL == 1 ? add(n1, B) : n1 * splat(MUL) + B;
+        }
+    }
+    // new row-above input: lo <- hi of lane-1, hi <- own lo (lane_up).
+    // The first lane of a task takes `first` (the boundary value) in the
+    // high half.
+    template <int L> __device__ static inline V shuffle_up(V out, int first,
+                                                           bool fl)
     {
         uint32_t o = u(out);
-        uint32_t sh = (uint32_t)__builtin_amdgcn_update_dpp(
-            (int)((uint32_t)first << 16), (int)o, CTRL, 0xf, 0xf, false);
+        uint32_t sh = (uint32_t)lane_up<L>((int)((uint32_t)first << 16),
+                                           (int)o, fl);
         return v(__builtin_amdgcn_alignbit(o, sh, 16));
     }
     __device__ static inline V set_lo(V a, int x)
@@ -190,11 +229,21 @@ template <> struct Ops<false> {
             s[r] = I2{(int)(int8_t)(lw[r / 4] >> (8 * (r % 4))),
                       (int)(int8_t)(hw[r / 4] >> (8 * (r % 4)))};
     }
-    template <int CTRL> __device__ static inline V shuffle_up(V out, int first)
+    template <int MUL>
+    __device__ static inline void unpack_nib(uint32_t lw, uint32_t hw,
+                                             int base, V *s)
     {
-        int sh = __builtin_amdgcn_update_dpp(first, out.y, CTRL, 0xf, 0xf,
-                                             false);
-        return I2{sh, out.x};
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const int sh = 8 * (r % 4) + 4 * (r / 4);
+            s[r] = I2{(int)((lw >> sh) & 15u) * MUL + base,
+                      (int)((hw >> sh) & 15u) * MUL + base};
+        }
+    }
+    template <int L> __device__ static inline V shuffle_up(V out, int first,
+                                                           bool fl)
+    {
+        return I2{lane_up<L>(first, out.y, fl), out.x};
     }
     __device__ static inline V set_lo(V a, int x)
     {
@@ -282,35 +331,50 @@ template <int L> __device__ inline uint64_t group_max_u64(uint64_t v)
     return v;
 }
 
-// L lanes per task: 64 (one task per wave) or 16 (four tasks per wave, each
-// on one 16-lane DPP row, in lockstep; the wave runs as many passes as its
-// longest task needs, and each slot keeps its own boundary row).
+// L lanes per task: 64 (one task per wave), or 32, 16 or 8 (G = 64 / L tasks
+// per wave on aligned lane groups, in lockstep; the wave runs as many passes
+// as its longest task needs, and each slot keeps its own boundary row).
+//
+// Kernels with 8 rows per lane keep 4-bit query profiles (rows_nib), half
+// the LDS of byte profiles, and are compiled for three waves per SIMD, so
+// that twelve one-wave workgroups fit a CU.
 template <int R, int L, bool LOCAL, bool TRACE, bool I16>
-__global__ __launch_bounds__(64) void k_dp(DpArgs a)
+__global__ __launch_bounds__(64)
+PA_WAVES_PER_EU(rows_nib(R) && I16 ? 3 : 1)
+void k_dp(DpArgs a)
 {
     typedef Ops<I16> O;
     typedef typename O::V V;
     typedef typename O::Key Key;
-    static_assert(L == 64 || L == 16, "lanes per task");
+    static_assert(L == 64 || L == 32 || L == 16 || L == 8, "lanes per task");
     static_assert(R == 4 || R == 8, "rows per lane");
+    constexpr bool NIB = rows_nib(R);
     constexpr int G = 64 / L;                 // tasks per wave
     constexpr int ROWS = 2 * R * L;           // rows per pass
-    constexpr int SHR = L == 64 ? 0x138 : 0x111;   // wave_shr:1 / row_shr:1
+    // 32-bit words of one residue code's profile: 4 rows (bytes) or 8 rows
+    // (nibbles) per word
+    constexpr int PW = NIB ? ROWS / 8 : ROWS / 4;
+    // columns per residue block: one per lane (L = 64) or four per lane
+    constexpr int RB = L == 64 ? 64 : 4 * L;
     constexpr int SH = TRACE ? 2 : 0;
     constexpr int NEG = TRACE ? O::kNegTraced : O::kNeg;
     HIP_DYNAMIC_SHARED(uint32_t, prof32)
     const int lane = threadIdx.x;
     const int gl = lane % L;     // lane within the task
     const int grp = lane / L;    // task slot of the wave
+    const bool fl = gl == 0;     // first lane of its task
     // query profile of this lane's task
-    uint32_t *gprof32 = prof32 + grp * (kProfChars * ROWS / 4);
+    uint32_t *gprof32 = prof32 + grp * (kProfChars * PW);
     const uint8_t *prof = (const uint8_t *)gprof32;
     // profile matrix of this mode in global memory (1 KB, cached): the score,
     // or 4*score+2 when traced, as int8, -128 outside the alphabet (row 31:
     // padding rows).  Global traces tag a diagonal step between equal
     // residues 3 instead of 2 (4*score+3): the traceback counts identities
     // from the tags, and diagonal still wins every tie against F and E.  The
-    // LDS holds the query profiles only.
+    // LDS holds the query profiles only.  4-bit profiles (NIB) hold score -
+    // nib_min (0 outside the alphabet, the lowest score: padding rows stay
+    // below every real cell) and tag every diagonal 2.
+    const int nbase = TRACE ? 4 * a.nib_min + 2 : a.nib_min;
     const uint32_t count = *a.count;
     const int open = a.gap_open * (1 << SH);
     const int ext = a.gap_ext * (1 << SH);
@@ -381,7 +445,10 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
             __syncthreads();
             // query profile of rows [i0, i0 + ROWS): each lane writes its own
             // 2R rows, four rows (one dword per residue code) at a time,
-            // gathering bytes of four matrix rows with v_perm
+            // gathering bytes of four matrix rows with v_perm; 4-bit
+            // profiles put rows 4-7 of a strip in the high nibbles of rows
+            // 0-3's bytes (one dword per strip and code)
+            uint32_t xs[NIB ? kProfChars : 1];
 #pragma unroll
             for (int g = 0; g < R / 2; ++g) {
                 const int row0 = i0 + 2 * R * gl + 4 * g;
@@ -410,8 +477,17 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
                         __builtin_amdgcn_perm(mr[1][wd], mr[0][wd], sel);
                     const uint32_t x23 =
                         __builtin_amdgcn_perm(mr[3][wd], mr[2][wd], sel);
-                    dst[c * (ROWS / 4)] =
+                    const uint32_t x =
                         __builtin_amdgcn_perm(x23, x01, 0x05040100u);
+                    if constexpr (NIB) {
+                        if ((g & 1) == 0)
+                            xs[c] = x;
+                        else
+                            gprof32[c * PW + 2 * gl + (g >> 1)] =
+                                xs[c] | (x << 4);
+                    } else {
+                        dst[c * PW] = x;
+                    }
                 }
             }
             if (p > 0)
@@ -446,8 +522,8 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
                 bh1 = bnd[2];
                 bf1 = bnd[3];
             }
-            V upH = O::template shuffle_up<SHR>(outH, bh1);
-            V upF = O::template shuffle_up<SHR>(outF, bf1);
+            V upH = O::template shuffle_up<L>(outH, bh1, fl);
+            V upF = O::template shuffle_up<L>(outF, bf1, fl);
             V dg = O::splat(NEG);
             dg = gl == 0 ? O::set_lo(dg, bh0) : dg;
             int c_lo = kSentinel, c_hi = kSentinel;
@@ -481,25 +557,28 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
             if (L == 64)   // one task per wave: keep it in a scalar register
                 seg_skip = __builtin_amdgcn_readfirstlane(seg_skip);
             // Column residues and (in later passes) the boundary row reach
-            // lane 0 through v_readlane / a shuffle from registers that are
-            // filled a block ahead: lane l holds column base + l (64-lane
-            // tasks) or columns base + 4l .. base + 4l + 3 (16-lane tasks)
-            // of the residues, and column base + l of the boundary row, with
-            // base = 1 (mod block).  The blocks switch at steps t = 0 (mod L)
-            // outside the step loop, so no step waits on global memory.  The
-            // boundary row of this pass is written 128 steps after it is
-            // read, so the look-ahead never reads a value of this pass.
+            // a task's first lane through v_readlane / a shuffle from
+            // registers that are filled a block ahead: lane l holds column
+            // base + l (64-lane tasks) or columns base + 4l .. base + 4l + 3
+            // (narrower tasks) of a block of RB residues, and column base + l
+            // of a block of L boundary values, with base = 1 (mod block).
+            // The blocks switch at steps t = 0 (mod RB or L) outside the step
+            // loop, so no step waits on global memory.  The boundary row of
+            // this pass is written 2L - 1 steps after the column is read
+            // (the last lane lags that far), and the look-ahead only reads
+            // columns ahead of the step, so it never reads a value of this
+            // pass.
             auto res_at = [&](int j) {   // residue of column j (1-based)
                 return j <= tlen ? (int)ts[j - 1] : kSentinel;
             };
-            auto res_blk = [&](int base) {   // block of 64 columns from base
+            auto res_blk = [&](int base) {   // block of RB columns from base
                 if (L == 64)
                     return res_at(base + lane);
                 const int c = base + 4 * gl;
                 return res_at(c) | (res_at(c + 1) << 8) |
                        (res_at(c + 2) << 16) | (res_at(c + 3) << 24);
             };
-            int rc_cur = res_blk(1), rc_nxt = res_blk(65);
+            int rc_cur = res_blk(1), rc_nxt = res_blk(RB + 1);
             int bh_cur = 0, bf_cur = NEGF, bh_nxt = 0, bf_nxt = NEGF;
             if (p > 0) {
                 if (gl + 1 <= tlen) {
@@ -516,7 +595,7 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
             // previous step (two columns back for them); no lane-dependent
             // branch surrounds the DPP move.
             auto next_res = [&](int tn) {
-                const int ri = (tn - 1) & 63;
+                const int ri = (tn - 1) & (RB - 1);
                 int c0;
                 if (L == 64) {
                     c0 = __builtin_amdgcn_readlane(rc_cur, ri);
@@ -526,16 +605,26 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
                 }
                 // the first lane of a task has no source lane and keeps the
                 // DPP "old" operand, column tn's residue
-                const int nc = __builtin_amdgcn_update_dpp(c0, c_hi, SHR, 0xf,
-                                                           0xf, false);
+                const int nc = lane_up<L>(c0, c_hi, fl);
                 c_hi = c_lo;
                 c_lo = nc;
+            };
+            // profile words of the current residues: R bytes of the lo and
+            // the hi strip, or one word of nibbles each
+            auto load_prof = [&](uint32_t *lw, uint32_t *hw) {
+                if constexpr (NIB) {
+                    lw[0] = gprof32[c_lo * PW + 2 * gl];
+                    hw[0] = gprof32[c_hi * PW + 2 * gl + 1];
+                } else {
+                    ProfWords<R>::load(prof + c_lo * ROWS + 2 * R * gl, lw);
+                    ProfWords<R>::load(prof + c_hi * ROWS + 2 * R * gl + R,
+                                       hw);
+                }
             };
             // residues and profile words of step 1
             next_res(1);
             uint32_t lwA[R / 4], hwA[R / 4], lwB[R / 4], hwB[R / 4];
-            ProfWords<R>::load(prof + c_lo * ROWS + 2 * R * gl, lwA);
-            ProfWords<R>::load(prof + c_hi * ROWS + 2 * R * gl + R, hwA);
+            load_prof(lwA, hwA);
             // The step loop, specialised for the first pass (boundary row
             // from the gap model) and later passes (boundary row from the
             // previous pass).  Steps run in blocks of L between the block
@@ -549,9 +638,7 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
                 auto step = [&](int t, const uint32_t *lw, const uint32_t *hw,
                                 uint32_t *lwn, uint32_t *hwn) {
                     next_res(t + 1);
-                    ProfWords<R>::load(prof + c_lo * ROWS + 2 * R * gl, lwn);
-                    ProfWords<R>::load(prof + c_hi * ROWS + 2 * R * gl + R,
-                                       hwn);
+                    load_prof(lwn, hwn);
                     int nbh = 0, nbf = NEGF;  // boundary for lane 0 at step t+1
                     {
                         const int jn = t + 1;
@@ -578,7 +665,11 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
                     // t_last of this lane
                     if (t >= t_first && t <= t_last) {
                         V s[R];
-                        O::template unpack<R>(lw, hw, s);
+                        if constexpr (NIB)
+                            O::template unpack_nib<TRACE ? 4 : 1>(lw[0], hw[0],
+                                                                  nbase, s);
+                        else
+                            O::template unpack<R>(lw, hw, s);
                         V hdiag = dg, hup = upH, fup = upF;
                         V hoUp = O::add(upH, OPEN);
                         V cm = O::splat(0);
@@ -663,8 +754,8 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
                         }
                     }
                     dg = upH;
-                    upH = O::template shuffle_up<SHR>(outH, nbh);
-                    upF = O::template shuffle_up<SHR>(outF, nbf);
+                    upH = O::template shuffle_up<L>(outH, nbh, fl);
+                    upF = O::template shuffle_up<L>(outF, nbf, fl);
                     if (TRACE) {
                         trp += R;
                         if (t % (int)kTraceSeg == 0)   // wave-uniform
@@ -674,10 +765,10 @@ __global__ __launch_bounds__(64) void k_dp(DpArgs a)
                 for (int tb = 0; tb <= last_step; tb += L) {
                     if (tb > 0) {
                         // block switch before step tb (tb = 0 mod L): the
-                        // residues every 64 steps, the boundary row every L
-                        if ((tb & 63) == 0) {
+                        // residues every RB steps, the boundary row every L
+                        if (tb % RB == 0) {
                             rc_cur = rc_nxt;
-                            rc_nxt = res_blk(tb + 65);
+                            rc_nxt = res_blk(tb + RB + 1);
                         }
                         if constexpr (!FIRST) {
                             bh_cur = bh_nxt;
@@ -785,8 +876,10 @@ namespace {
 
 // R and L are fixed per launch (one launch per DP class, whose kernel
 // shape wrote the trace), so the layout arithmetic divides by constants.
-// Global traces mark a diagonal step between equal residues with tag 3
-// (restart does not occur there), so LOCAL = false needs no residue loads.
+// Global traces of byte-profile kernels mark a diagonal step between equal
+// residues with tag 3 (restart does not occur there), so LOCAL = false needs
+// no residue loads; those of 4-bit-profile kernels (rows_nib) tag every
+// diagonal 2 and the walk compares the residues, as local walks do.
 //
 // STAGE: the walk reads the trace one lane run at a time: the kTraceSeg
 // records of one lane in one segment (8R contiguous bytes, 8R-aligned) are
@@ -796,6 +889,26 @@ namespace {
 // so a run serves several steps.  Without STAGE each step loads its byte
 // from global memory.
 constexpr int kTbThreads = 256;
+
+// Residues of one sequence as the walk reads them (backwards, mostly one
+// position per diagonal step): the aligned 8-byte word holding the last one
+// read is kept, so a word load serves several steps.  The sequence store is
+// allocated with slack behind its last residue.
+struct ResWin {
+    const uint8_t PA_GLOBAL *base;
+    uintptr_t word = 1;   // address of the word held (1: none)
+    uint64_t v = 0;
+    __device__ inline int at(int i)
+    {
+        const uintptr_t p = (uintptr_t)(base + i);
+        const uintptr_t wa = p & ~(uintptr_t)7;
+        if (wa != word) {
+            word = wa;
+            v = *(const uint64_t PA_GLOBAL *)wa;
+        }
+        return (int)((v >> (8u * (uint32_t)(p & 7))) & 0xffu);
+    }
+};
 
 template <int R, int L, bool LOCAL, bool STAGE>
 __global__ __launch_bounds__(kTbThreads) void k_traceback(TbArgs a)
@@ -856,6 +969,7 @@ __global__ __launch_bounds__(kTbThreads) void k_traceback(TbArgs a)
         }
     };
     uint32_t cur_run = 0xffffffffu;   // byte offset of the staged run
+    ResWin qwin{qs}, twin{ts};
     int matches = 0, cols = 0, runs = 0;
     int last = 0;   // 1 diagonal, 2 vertical, 3 horizontal
     int state = 0;  // 0 H, 1 E (horizontal), 2 F (vertical)
@@ -912,8 +1026,8 @@ __global__ __launch_bounds__(kTbThreads) void k_traceback(TbArgs a)
                     break;  // restart
                 if (src >= 2) {
                     ++cols;
-                    if (LOCAL)
-                        matches += qs[i - 1] == ts[j - 1];
+                    if (LOCAL || rows_nib(R))
+                        matches += qwin.at(i - 1) == twin.at(j - 1);
                     else
                         matches += src == 3;
                     last = 1;
@@ -950,8 +1064,8 @@ __global__ __launch_bounds__(kTbThreads) void k_traceback(TbArgs a)
 template <int R, int L, bool LC, bool T, bool I>
 hipError_t launch_one(const DpArgs &a, int grid, hipStream_t s)
 {
-    // G task profiles of 2RL rows each (128R rows in all)
-    const size_t lds = (size_t)kProfChars * 128 * R;
+    // G task profiles of 2RL rows each (128R rows in all), bytes or nibbles
+    const size_t lds = (size_t)kProfChars * (rows_nib(R) ? 64 : 128) * R;
     hipLaunchKernelGGL((k_dp<R, L, LC, T, I>), dim3(grid), dim3(64), lds, s,
                        a);
     return hipGetLastError();
@@ -960,7 +1074,7 @@ hipError_t launch_one(const DpArgs &a, int grid, hipStream_t s)
 template <int R, int L, bool LC, bool T, bool I>
 int occupancy_one()
 {
-    const size_t lds = (size_t)kProfChars * 128 * R;
+    const size_t lds = (size_t)kProfChars * (rows_nib(R) ? 64 : 128) * R;
     int n = 0;
     hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_dp<R, L, LC, T, I>, 64,
                                                  lds);
@@ -1026,6 +1140,8 @@ hipError_t launch_dp(int shape, bool local, bool trace, bool i16,
     case kShapeL16R8: return launch_c<8, 16>(local, trace, i16, a, grid, s);
     case kShapeL64R4: return launch_c<4, 64>(local, trace, i16, a, grid, s);
     case kShapeL64R8: return launch_c<8, 64>(local, trace, i16, a, grid, s);
+    case kShapeL32R8: return launch_c<8, 32>(local, trace, i16, a, grid, s);
+    case kShapeL8R8: return launch_c<8, 8>(local, trace, i16, a, grid, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -1041,7 +1157,9 @@ int dp_occupancy(int shape, bool local, bool trace, bool i16)
             shape == kShapeL16R4   ? occupancy_c<4, 16>(local, trace, i16)
             : shape == kShapeL16R8 ? occupancy_c<8, 16>(local, trace, i16)
             : shape == kShapeL64R4 ? occupancy_c<4, 64>(local, trace, i16)
-                                   : occupancy_c<8, 64>(local, trace, i16);
+            : shape == kShapeL64R8 ? occupancy_c<8, 64>(local, trace, i16)
+            : shape == kShapeL32R8 ? occupancy_c<8, 32>(local, trace, i16)
+                                   : occupancy_c<8, 8>(local, trace, i16);
     return cache[shape][vi];
 }
 
@@ -1053,6 +1171,8 @@ hipError_t launch_traceback(const TbArgs &a, int shape, bool stage, int grid,
     case kShapeL16R8: return launch_traceback_c<8, 16>(a, stage, grid, s);
     case kShapeL64R4: return launch_traceback_c<4, 64>(a, stage, grid, s);
     case kShapeL64R8: return launch_traceback_c<8, 64>(a, stage, grid, s);
+    case kShapeL32R8: return launch_traceback_c<8, 32>(a, stage, grid, s);
+    case kShapeL8R8: return launch_traceback_c<8, 8>(a, stage, grid, s);
     default: return hipErrorInvalidValue;
     }
 }

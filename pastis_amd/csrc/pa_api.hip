@@ -93,7 +93,6 @@ enum {
 };
 
 constexpr int kMaxWgPerCu = 32;     // one-wave workgroups per CU (8 per SIMD)
-constexpr int kSlotsPerWg = 4;      // task slots of a 16-lane-group wave
 constexpr int kXdWideLanes = 65536;  // lanes of the wide (global scratch) re-run
 constexpr int kXdGraphLanes = 4096;  // its lanes in a captured batch
 
@@ -188,17 +187,24 @@ int ensure_common(pa_ctx *c, uint64_t nt)
 
 // Persistent DP grid: occupancy-limited workgroups per CU, capped by the
 // tuning and by the number of tasks.
-int dp_grid(pa_ctx *c, uint64_t ntasks, int cls, bool local, bool trace,
-            bool i16)
+// persistent DP workgroups per CU of a class's kernel
+int dp_wg_per_cu(pa_ctx *c, int cls, bool local, bool trace, bool i16)
 {
     int per_cu = dp_occupancy(c->shapes.shape[cls], local, trace, i16);
     if (c->tune.dp_wg_per_cu > 0)
         per_cu = std::min(per_cu, (int)c->tune.dp_wg_per_cu);
-    if (c->wg_cap > 0)
-        per_cu = std::min(per_cu, c->wg_cap);
-    per_cu = std::min(per_cu, kMaxWgPerCu);
-    return (int)std::max<uint64_t>(
-        1, std::min<uint64_t>(ntasks, (uint64_t)c->ncu * per_cu));
+    return std::min(per_cu, kMaxWgPerCu);
+}
+
+int dp_grid(pa_ctx *c, uint64_t ntasks, int cls, bool local, bool trace,
+            bool i16)
+{
+    uint64_t g = std::min<uint64_t>(
+        ntasks, (uint64_t)c->ncu * dp_wg_per_cu(c, cls, local, trace, i16));
+    if (c->slot_cap > 0)
+        g = std::min<uint64_t>(
+            g, c->slot_cap / (uint64_t)shape_slots(c->shapes.shape[cls]));
+    return (int)std::max<uint64_t>(1, g);
 }
 
 int xd_grid(pa_ctx *c, uint64_t nitems)
@@ -211,14 +217,20 @@ int xd_grid(pa_ctx *c, uint64_t nitems)
         1, std::min<uint64_t>((nitems + 63) / 64, (uint64_t)c->ncu * per_cu));
 }
 
-int ensure_bnd(pa_ctx *c)
+int ensure_bnd(pa_ctx *c, bool local, bool trace)
 {
-    // one boundary row per task slot of each persistent workgroup the DP
-    // grids can hold (64 / 16 slots per one-wave workgroup)
-    const int wg = c->wg_cap > 0 ? std::min(c->wg_cap, kMaxWgPerCu)
-                                 : kMaxWgPerCu;
-    const size_t need = (size_t)c->ncu * wg * kSlotsPerWg * 2u *
-                        (size_t)(c->max_clen + 2) * sizeof(int32_t);
+    // one boundary row per task slot of the persistent workgroups of the
+    // run's DP grids: per class, its grid (dp_grid) times the task slots of
+    // its shape's one-wave workgroup
+    size_t slots = 1;
+    for (int cls = 0; cls < kClasses; ++cls)
+        for (int i16 = 0; i16 < 2; ++i16)
+            slots = std::max(slots,
+                             (size_t)dp_grid(c, ~0ull, cls, local, trace,
+                                             i16 != 0) *
+                                 shape_slots(c->shapes.shape[cls]));
+    const size_t need =
+        slots * 2u * (size_t)(c->max_clen + 2) * sizeof(int32_t);
     return grow(c, c->bnd, need);
 }
 
@@ -236,7 +248,7 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
            hipStream_t s)
 {
     int rc;
-    if ((rc = ensure_bnd(c)))
+    if ((rc = ensure_bnd(c, local, trace)))
         return rc;
     // a fill kernel, not a memset node, so that the sequence can be captured
     HIPCHK(c, fill_u32(P<uint32_t>(c->tflags), 0, (uint32_t)((nt + 3) / 4), s));
@@ -365,10 +377,13 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
         args.codes = nullptr;  // task offsets hold absolute device addresses
         args.tasks = P<DpTask>(c->tasks);
         args.mat = P<int8_t>(c->dmat);
-        args.pmat = (const uint32_t *)(P<int8_t>(c->dmat) +
-                                       (!local ? kPmatGlobal
-                                               : trace ? kPmatLocal
-                                                       : kPmatUntraced));
+        const uint32_t *pmat_byte =
+            (const uint32_t *)(P<int8_t>(c->dmat) +
+                               (!local ? kPmatGlobal
+                                       : trace ? kPmatLocal : kPmatUntraced));
+        const uint32_t *pmat_nib =
+            (const uint32_t *)(P<int8_t>(c->dmat) + kPmatNib);
+        args.nib_min = c->nib_min;
         args.out = P<DpOut>(c->dpout);
         args.trace = trace ? P<uint8_t>(c->trace) - base : nullptr;
         args.trace_off = trace ? P<uint64_t>(c->toffs) : nullptr;
@@ -406,6 +421,9 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
                 // short tasks are claimed many per atomic, long ones singly
                 // (the 16-lane classes claim rounds of four tasks)
                 args.chunk = rcl <= 2 ? 4u : 1u;
+                args.pmat = rows_nib(shape_rows(c->shapes.shape[rcl]))
+                                ? pmat_nib
+                                : pmat_byte;
                 for (int w = 0; w < 2; ++w) {
                     const int bin = 2 * rcl + w;
                     args.list = lists + (size_t)bin * nt;
@@ -927,6 +945,23 @@ int pa_create(const pa_config *cfg, pa_ctx **out)
             dm[kPmatLocal + q * 32 + x] = (int8_t)(4 * sc + 2);
             dm[kPmatGlobal + q * 32 + x] = (int8_t)(4 * sc + 2 + (q == x));
         }
+    // 4-bit profiles: scores over the alphabet within 16 values of a
+    // negative minimum; nibble 0 (the minimum) outside the alphabet
+    int lo = 127, hi = -128;
+    for (int q = 0; q < kAlpha; ++q)
+        for (int x = 0; x < kAlpha; ++x) {
+            lo = std::min(lo, (int)c->mat[q * 32 + x]);
+            hi = std::max(hi, (int)c->mat[q * 32 + x]);
+        }
+    c->nib_ok = lo < 0 && hi - lo <= 15;
+    c->nib_min = lo;
+    for (int q = 0; q < 32; ++q)
+        for (int x = 0; x < 32; ++x)
+            dm[kPmatNib + q * 32 + x] =
+                (int8_t)(c->nib_ok && q < kAlpha && x < kAlpha
+                             ? c->mat[q * 32 + x] - lo
+                             : 0);
+    c->shapes = shapes_for(c, c->tune);
     if (grow(c, c->dmat, kPmatBytes) ||
         hipMemcpy(c->dmat.p, dm.data(), kPmatBytes, hipMemcpyHostToDevice) !=
             hipSuccess) {
@@ -1021,7 +1056,7 @@ int pa_set_tuning(pa_ctx *c, const pa_tuning *t)
         if (t->dp_shape[k] < 0 || t->dp_shape[k] >= kShapes)
             return fail(c, PA_ERR_ARG, "dp_shape outside pa_dp_shape");
     c->tune = *t;
-    c->shapes = shapes_of(*t);
+    c->shapes = shapes_for(c, *t);
     ++c->gen;   // grids and class shapes of captured graphs follow the tuning
     return PA_OK;
 }
@@ -1209,7 +1244,7 @@ int pa_sw_scores_graph(pa_ctx *c, const pa_pair *d_pairs, uint64_t n,
     if (!hit) {
         // every buffer the batch uses exists before the capture starts; a
         // reallocation here invalidates the other graphs
-        if ((rc = ensure_common(c, n)) || (rc = ensure_bnd(c)))
+        if ((rc = ensure_common(c, n)) || (rc = ensure_bnd(c, true, false)))
             return rc;
         for (size_t i = 0; i < G.size();) {
             if (G[i].gen != c->gen || (G.size() >= (size_t)pa_ctx::kGraphs &&
@@ -1610,8 +1645,8 @@ int pa_get_kernel_stats(pa_ctx *c, pa_kernel_stat *out, int cap, int *n)
                 std::memset(&o, 0, sizeof(o));
                 // mode, row class, the class's current kernel shape
                 // (lanes x rows per lane), arithmetic
-                static const char *shname[kShapes] = {"l16r4", "l16r8",
-                                                      "l64r4", "l64r8"};
+                static const char *shname[kShapes] = {
+                    "l16r4", "l16r8", "l64r4", "l64r8", "l32r8", "l8r8"};
                 const int cls = b < kBins ? b / 2 : b - kBins;
                 const char *sh = shname[c->shapes.shape[cls]];
                 if (b < kBins)

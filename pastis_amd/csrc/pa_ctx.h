@@ -130,14 +130,29 @@ struct pa_ctx {
     uint64_t gen = 0;
     pa_tuning tune = pa::default_tuning();
     pa::DpShapes shapes = pa::kDefaultShapes;   // derived from tune
+    // the scoring matrix fits 4-bit profiles (kPmatNib) with scores nib_min
+    // .. nib_min + 15; if not, shapes replaces the 4-bit-profile shapes
+    bool nib_ok = true;
+    int nib_min = 0;
     // No event timers (pa_sw_scores_noalloc: event objects are allocated).
     bool quiet = false;
-    // Cap on persistent DP workgroups per CU (0 = none); the boundary-row
-    // store is sized for it.
-    int wg_cap = 0;
+    // Cap on the task slots (boundary rows) of a DP grid (0 = none); the
+    // boundary-row store is sized for the grids.
+    uint64_t slot_cap = 0;
 };
 
 namespace pa {
+
+// kernel shapes of a tuning on this context (4-bit-profile shapes replaced
+// when the matrix does not fit them)
+inline DpShapes shapes_for(const pa_ctx *c, const pa_tuning &t)
+{
+    DpShapes sh = shapes_of(t);
+    if (!c->nib_ok)
+        for (int k = 0; k < 4; ++k)
+            sh.shape[k] = (uint8_t)shape_byte_fallback(sh.shape[k]);
+    return sh;
+}
 
 inline int fail(pa_ctx *c, int code, const std::string &msg)
 {

@@ -17,6 +17,10 @@ namespace pa {
 #ifndef PA_GLOBAL   // the CPU emulation build of the tests defines it empty
 #define PA_GLOBAL __attribute__((address_space(1)))
 #endif
+// Waves per SIMD a kernel's register allocation must allow.
+#ifndef PA_WAVES_PER_EU   // the CPU emulation build of the tests: empty
+#define PA_WAVES_PER_EU(n) __attribute__((amdgpu_waves_per_eu(n)))
+#endif
 template <class T> __device__ inline T PA_GLOBAL *glb(T *p)
 {
     return (T PA_GLOBAL *)p;
@@ -63,8 +67,17 @@ constexpr uint32_t kTaskSat = 1u;  // int16 saturated, needs the int32 re-run
 // (traced local) or 4*score+2, +1 on the diagonal (traced global: a
 // diagonal step between equal residues carries tag 3).  Byte offsets inside
 // the device copy of the scoring matrix, after the plain 32x32 table.
-constexpr size_t kPmatUntraced = 1024, kPmatLocal = 2048, kPmatGlobal = 3072;
-constexpr size_t kPmatBytes = 4096;
+//
+// The kernels with 8 rows per lane (rows_nib) hold 4-bit query profiles:
+// kPmatNib gives score - nib_min in 0..15 (0 outside the alphabet), and
+// the kernel adds nib_min (untraced) or 4*nib_min + 2 (traced) back; their
+// global traces tag every diagonal step 2 and the traceback compares the
+// residues for identities.  They need a matrix whose scores over the 25x25
+// alphabet span at most 16 values with a negative minimum (BLOSUM62: -4 ..
+// 11); with any other matrix the context runs their 4-row counterparts.
+constexpr size_t kPmatUntraced = 1024, kPmatLocal = 2048, kPmatGlobal = 3072,
+                 kPmatNib = 4096;
+constexpr size_t kPmatBytes = 5120;
 // Global task whose scores are bounded by the residue bound below (no cell
 // exceeds min over the two spans of the sum of each residue's best score),
 // small enough for the traced int16 kernels.
@@ -98,6 +111,7 @@ struct DpArgs {
     uint32_t *sat_count;
     int32_t gap_open, gap_ext;
     uint32_t chunk;            // tasks claimed per queue atomic (>= 1)
+    int32_t nib_min;           // 4-bit profiles: the score of nibble 0
 };
 
 // Arguments of the traceback kernel.
@@ -116,12 +130,38 @@ struct TbArgs {
 // DP kernel shapes: L lanes per task (G = 64 / L tasks per wave, one per
 // L-lane group) and R rows per lane per half, so a pass covers 2RL rows and
 // longer tasks take several passes linked by a boundary row.  Every shape
-// runs any row count; the kernels are compiled for these four.
-constexpr int kShapes = 4;
+// runs any row count; the kernels are compiled for these six.
+constexpr int kShapes = 6;
 constexpr int kShapeL16R4 = 0, kShapeL16R8 = 1, kShapeL64R4 = 2,
-              kShapeL64R8 = 3;
-__host__ __device__ constexpr int shape_rows(int sh) { return (sh & 1) ? 8 : 4; }
-__host__ __device__ constexpr int shape_lanes(int sh) { return (sh & 2) ? 64 : 16; }
+              kShapeL64R8 = 3, kShapeL32R8 = 4, kShapeL8R8 = 5;
+__host__ __device__ constexpr int shape_rows(int sh)
+{
+    return sh == kShapeL16R4 || sh == kShapeL64R4 ? 4 : 8;
+}
+__host__ __device__ constexpr int shape_lanes(int sh)
+{
+    return sh == kShapeL64R4 || sh == kShapeL64R8 ? 64
+           : sh == kShapeL32R8                    ? 32
+           : sh == kShapeL8R8                     ? 8
+                                                  : 16;
+}
+// task slots (boundary rows) of a one-wave DP workgroup
+__host__ __device__ constexpr int shape_slots(int sh)
+{
+    return 64 / shape_lanes(sh);
+}
+// 4-bit query profiles: the shapes with 8 rows per lane
+__host__ __device__ constexpr bool rows_nib(int R) { return R == 8; }
+// the shape that replaces a 4-bit-profile shape when the matrix does not
+// fit 4 bits (same rows per pass where one exists)
+__host__ __device__ constexpr int shape_byte_fallback(int sh)
+{
+    return sh == kShapeL16R8   ? kShapeL16R4
+           : sh == kShapeL64R8 ? kShapeL64R4
+           : sh == kShapeL32R8 ? kShapeL64R4
+           : sh == kShapeL8R8  ? kShapeL16R4
+                               : sh;
+}
 
 // DP row classes: a task's class is the first c with qlen <= edge[c] (class
 // 3 takes the rest), and each class has its kernel shape.  Narrow groups

@@ -26,7 +26,6 @@ namespace {
 constexpr uint64_t kMaxPairs = 262144;     // pairs per call
 constexpr uint32_t kMaxLen = 4096;         // longest loaded sequence
 constexpr int kMaxSlots = 1024;            // boundary rows of kMaxLen
-constexpr int kSlots = 4;                  // task slots per DP workgroup
 
 constexpr size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
@@ -75,16 +74,12 @@ public:
             bufs[i]->cap = ends[i] - offs[i];
         }
         quiet_ = c->quiet;
-        wg_cap_ = c->wg_cap;
+        slot_cap_ = c->slot_cap;
         c->quiet = true;
-        // as many persistent workgroups per CU as the boundary-row region
-        // holds for the loaded sets' longest column sequence
+        // the DP grids hold no more task slots than the boundary-row region
+        // has rows of the loaded sets' longest column sequence
         const size_t row = 2u * ((size_t)c->max_clen + 2u) * sizeof(int32_t);
-        const size_t slots = kBndBytes / row;
-        const int per_cu = (int)std::max<size_t>(
-            1, std::min<size_t>(32, slots / ((size_t)kSlots *
-                                             std::max(1, c->ncu))));
-        c->wg_cap = wg_cap_ > 0 ? std::min(wg_cap_, per_cu) : per_cu;
+        c->slot_cap = kBndBytes / row;
     }
     ~ScratchView()
     {
@@ -93,14 +88,14 @@ public:
         for (int i = 0; i < 7; ++i)
             *bufs[i] = saved_[i];
         c_->quiet = quiet_;
-        c_->wg_cap = wg_cap_;
+        c_->slot_cap = slot_cap_;
     }
 
 private:
     pa_ctx *c_;
     Buf saved_[7];
     bool quiet_;
-    int wg_cap_;
+    uint64_t slot_cap_;
 };
 
 }  // namespace

@@ -71,8 +71,9 @@ def run(driver, tmp_path, w, pairs, mode, xdrop=0, coop=0,
 
 # Non-default launch shapes: 64-lane kernels on short rows and 16-lane
 # multi-pass kernels (128 rows per pass) on long ones, class edges that send
-# most tasks through several passes, LDS-staged tracebacks.
-SHAPES = [None, (3, 2, 1, 0, 16, 40, 100, 1)]
+# most tasks through several passes, LDS-staged tracebacks; then the 32- and
+# 8-lane kernels (eight and two tasks per wave) with multi-pass tasks.
+SHAPES = [None, (3, 2, 1, 0, 16, 40, 100, 1), (4, 5, 4, 5, 16, 40, 100, 1)]
 
 
 @pytest.mark.parametrize("shapes", SHAPES)
@@ -95,7 +96,8 @@ def test_emulated_sfa_under_sanitizers(driver, tmp_path, shapes):
 # (xd_wave 2: the 256-column band kernel alone)
 @pytest.mark.parametrize("coop,shapes,model,xd_wave", [
     (0, None, "as_called", None), (1, None, "as_called", None),
-    (0, SHAPES[1], "as_called", None), (0, None, "as_documented", None),
+    (0, SHAPES[1], "as_called", None), (0, SHAPES[2], "as_called", None),
+    (0, None, "as_documented", None),
     (0, None, "as_documented", 2)])
 def test_emulated_sxa_under_sanitizers(driver, tmp_path, coop, shapes, model,
                                        xd_wave):

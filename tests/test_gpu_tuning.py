@@ -18,14 +18,17 @@ from pastis_amd.aligner import Aligner, config_for_cli
 
 pytestmark = pytest.mark.gpu
 
-# (dp_shape per class, dp_edge, tb_stage); shapes 0..3 = l16r4, l16r8,
-# l64r4, l64r8
+# (dp_shape per class, dp_edge, tb_stage); shapes 0..5 = l16r4, l16r8,
+# l64r4, l64r8, l32r8, l8r8
 LAYOUTS = [
     ((0, 1, 2, 3), (128, 256, 512), 0),      # the defaults
     ((0, 0, 0, 0), (16, 40, 100), 1),        # 128-row passes everywhere
     ((3, 3, 3, 3), (128, 256, 512), 0),      # 1024-row passes everywhere
     ((3, 2, 1, 0), (16, 40, 100), 1),        # wide on short, narrow on long
     ((1, 0, 3, 2), (200, 300, 1000), 1),
+    ((5, 1, 4, 3), (128, 256, 512), 0),      # 8- and 32-lane tasks
+    ((4, 5, 4, 5), (16, 40, 100), 1),        # their multi-pass tasks
+    ((5, 4, 5, 4), (16, 40, 100), 0),
 ]
 
 
@@ -118,10 +121,67 @@ def test_sxa_layouts_match_the_oracle(model):
         al.close()
 
 
+# Scoring matrices other than BLOSUM62: twice BLOSUM62 (-8 .. 22) and
+# BLOSUM62 + 4 (0 .. 15) do not fit the 4-bit profiles of the 8-rows-per-lane
+# kernels, which the context then replaces by their 4-row counterparts;
+# BLOSUM62 - 1 (-5 .. 10) fits with another minimum.
+MATRICES = {"x2": lambda m: 2 * m, "plus4": lambda m: m + 4,
+            "minus1": lambda m: m - 1}
+
+
+def _mat_cfg(cfg, m):
+    cfg.matrix = m.ctypes.data
+    return cfg
+
+
+@pytest.mark.parametrize("name", sorted(MATRICES))
+def test_other_matrices_match_the_oracle(name):
+    base = orc.blosum62().astype(np.int16)
+    m = np.ascontiguousarray(MATRICES[name](base).astype(np.int8))
+    w = small_config1(n_seq=80, n_pairs=300, seed=85)
+    codes, offs, pairs = _with_long(w, 85)
+    go, ge = -2, -11
+    ref_sw = orc.sw_batch(codes, offs, pairs["r"], pairs["c"], go, ge, mat=m)
+    ref_sfa = orc.sfa_batch(codes, offs, pairs["r"], pairs["c"],
+                            oracle_params("as_called"), mat=m)
+    w2 = small_config2(n_seq=120, n_pairs=300, seed=86)
+    ref_sxa = orc.sxa_batch(w2.codes, w2.offsets, w2.pairs["r"],
+                            w2.pairs["c"], w2.pairs["seed"],
+                            oracle_params("as_called"), mat=m)
+    for layout in (LAYOUTS[0], LAYOUTS[5], LAYOUTS[7]):
+        print("matrix", name, "layout", layout, flush=True)
+        al = Aligner(_mat_cfg(config_for_cli(-11, -2, 6, 0, 2, "as_called"),
+                              m))
+        _set(al, layout)
+        al.load_seqs(codes, offs)
+        got = al.sw_scores(pairs)
+        assert np.array_equal(got["score"], ref_sw[0]), layout
+        assert np.array_equal(got["r_end"].astype(np.int32), ref_sw[1])
+        assert np.array_equal(got["c_end"].astype(np.int32), ref_sw[2])
+        got = al.align(pairs)
+        for gf, rf in [("score", "score"), ("r_end", "v_end"),
+                       ("c_end", "h_end"), ("r_beg", "v_beg"),
+                       ("c_beg", "h_beg"), ("matches", "matches"),
+                       ("cols", "cols"), ("gap_opens", "gap_opens")]:
+            assert np.array_equal(got[gf].astype(np.int64),
+                                  ref_sfa[rf].astype(np.int64)), (layout, gf)
+        al.close()
+        al = Aligner(_mat_cfg(config_for_cli(-11, -2, 6, 49, 2, "as_called"),
+                              m))
+        _set(al, layout)
+        al.load_seqs(w2.codes, w2.offsets)
+        got = al.align(w2.pairs)
+        for gf in ("score", "matches", "cols", "gap_opens"):
+            assert np.array_equal(got[gf].astype(np.int64),
+                                  ref_sxa[gf].astype(np.int64)), (layout, gf)
+        assert np.array_equal(got["score_aln"], ref_sxa["score_aln"])
+        al.close()
+
+
 def test_bad_layouts_are_rejected():
     al = Aligner(config_for_cli(-11, -2, 6, 49, 2, "as_called"))
     for kw in [dict(dp_edge=(128, 128, 512)), dict(dp_edge=(0, 10, 20)),
-               dict(dp_edge=(1, 2, 70000)), dict(dp_shape=(0, 1, 2, 4)),
+               dict(dp_edge=(1, 2, 70000)), dict(dp_shape=(0, 1, 2, 6)),
                dict(tb_stage=2)]:
         with pytest.raises(_lib.PaError):
             al.set_tuning(**kw)
