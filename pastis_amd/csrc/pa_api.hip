@@ -253,6 +253,7 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
     // a fill kernel, not a memset node, so that the sequence can be captured
     HIPCHK(c, fill_u32(P<uint32_t>(c->tflags), 0, (uint32_t)((nt + 3) / 4), s));
     std::vector<std::pair<uint64_t, uint64_t>> chunks;
+    std::vector<uint64_t> chunk_base;   // trace offset of each chunk's first
     if (trace && c->g_chunks > 0) {
         // captured batch: the trace store (g_cap bytes) is split into
         // g_chunks chunks on the device; no host synchronisation
@@ -310,28 +311,45 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
         } else {
             if ((rc = grow(c, c->trace, c->trace_cap)))
                 return rc;
-            c->h_toffs.resize(nt + 1);
-            HIPCHK(c, hipMemcpyAsync(c->h_toffs.data(), c->toffs.p,
-                                     (nt + 1) * 8, hipMemcpyDeviceToHost, s));
-            HIPCHK(c, hipStreamSynchronize(s));
-            uint64_t a = 0;
-            while (a < nt) {
-                uint64_t b = a + 1;
-                if (c->h_toffs[b] - c->h_toffs[a] > c->trace_cap)
+            // cut into chunks of at most trace_cap bytes on the device
+            // (k_trace_chunks); only the ranges and bases come back
+            int nch = (int)std::min<uint64_t>(total / c->trace_cap + 2,
+                                              1u << 20);
+            for (;;) {
+                const size_t bytes = (size_t)nch * 16 + 4;
+                if ((rc = grow(c, c->gchunk, bytes)))
+                    return rc;
+                uint32_t *rg = P<uint32_t>(c->gchunk);
+                uint64_t *bs = (uint64_t *)(rg + 2 * nch);
+                uint32_t *err = (uint32_t *)(bs + nch);
+                HIPCHK(c, fill_u32(err, 0, 1, s));
+                HIPCHK(c, trace_chunks(P<uint64_t>(c->toffs), (uint32_t)nt,
+                                       c->trace_cap, nch, rg, bs, err, s));
+                std::vector<uint8_t> h(bytes);
+                HIPCHK(c, hipMemcpyAsync(h.data(), rg, bytes,
+                                         hipMemcpyDeviceToHost, s));
+                HIPCHK(c, hipStreamSynchronize(s));
+                const uint32_t *hr = (const uint32_t *)h.data();
+                const uint64_t *hb = (const uint64_t *)(hr + 2 * nch);
+                const uint32_t herr = *(const uint32_t *)(hb + nch);
+                // the chunker stops with the error flag either at a task
+                // larger than the store (the ranges then end at nt) or when
+                // the chunks run out before the tasks do
+                const bool ran_out = hr[2 * (nch - 1) + 1] < nt;
+                if (herr && !ran_out)
                     return fail(c, PA_ERR_LIMIT,
                                 "one alignment needs more trace memory than "
                                 "the trace store holds");
-                // largest b with toffs[b] - toffs[a] <= cap
-                uint64_t lo = b, hi = nt;
-                while (lo < hi) {
-                    uint64_t mid = (lo + hi + 1) / 2;
-                    if (c->h_toffs[mid] - c->h_toffs[a] <= c->trace_cap)
-                        lo = mid;
-                    else
-                        hi = mid - 1;
+                if (herr) {   // more chunks than estimated
+                    nch *= 2;
+                    continue;
                 }
-                chunks.push_back({a, lo});
-                a = lo;
+                for (int k = 0; k < nch; ++k)
+                    if (hr[2 * k] < hr[2 * k + 1]) {
+                        chunks.push_back({hr[2 * k], hr[2 * k + 1]});
+                        chunk_base.push_back(hb[k]);
+                    }
+                break;
             }
         }
     } else {
@@ -352,7 +370,7 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
         const uint32_t a = (uint32_t)ch.first, b = (uint32_t)ch.second;
         uint64_t base = 0;
         if (trace && !dev_chunks && chunks.size() > 1)
-            base = c->h_toffs[a];
+            base = chunk_base[ci];
         // device-sized chunk ci: task range and trace base from the device
         const uint32_t *rng =
             dev_chunks ? P<uint32_t>(c->gchunk) + 2 * ci : nullptr;

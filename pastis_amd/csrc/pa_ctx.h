@@ -86,7 +86,6 @@ struct pa_ctx {
     Buf pairs, results, scores, tasks, dpout, tflags, sizes, toffs, lists,
         sat, ctr, trace, bnd, xdt, xdo, ovf, cub, cells, dmat, ext;
     size_t trace_cap = 0;
-    std::vector<uint64_t> h_toffs;
     Buf bincells;                 // u64 cells per (DP mode, bin)
     Buf tri_flag, tri_pos;        // similarity-triple compaction
     struct KStat {
