@@ -540,8 +540,17 @@ void k_dp(DpArgs a)
             const int cap_lane = qrow / (2 * R);
             const int cap_half = (qrow % (2 * R)) / R;
             const int cap_r = qrow % R;
-            const int cap_step = tlen + 2 * cap_lane + cap_half;
-            const bool last_pass = p == npass_own - 1;   // this slot's
+            // values of the slot's task, in scalar registers when one task
+            // fills the wave, so that the per-step tests on them are scalar
+            auto uni = [](int x) {
+                if constexpr (L == 64)
+                    return __builtin_amdgcn_readfirstlane(x);
+                else
+                    return x;
+            };
+            const int cap_step = uni(tlen + 2 * cap_lane + cap_half);
+            const bool last_pass = p == uni(npass_own) - 1;   // this slot's
+            const bool more_passes = p + 1 < uni(npass_own);
 
             // trace record of (step 1, this lane); the next step's record
             // follows it, except after the last step of a segment, which
@@ -741,16 +750,20 @@ void k_dp(DpArgs a)
                                 for (int r = 0; r < R; ++r)
                                     saved[r] = O::blend(saved[r], H[r], ilo, ihi);
                             }
-                        } else if (last_pass && gl == cap_lane && t == cap_step) {
+                        } else if (last_pass && t == cap_step) {
+                            if (gl == cap_lane) {
     #pragma unroll
-                            for (int r = 0; r < R; ++r)
-                                if (r == cap_r)
-                                    gscore = cap_half ? O::hi(H[r]) : O::lo(H[r]);
+                                for (int r = 0; r < R; ++r)
+                                    if (r == cap_r)
+                                        gscore = cap_half ? O::hi(H[r])
+                                                          : O::lo(H[r]);
+                            }
                         }
-                        if (p + 1 < npass_own && gl == L - 1 && jhi >= 0 &&
-                            jhi <= tlen) {
-                            bnd[2 * jhi] = O::hi(outH);
-                            bnd[2 * jhi + 1] = O::hi(outF);
+                        if (more_passes) {
+                            if (gl == L - 1 && jhi >= 0 && jhi <= tlen) {
+                                bnd[2 * jhi] = O::hi(outH);
+                                bnd[2 * jhi + 1] = O::hi(outF);
+                            }
                         }
                     }
                     dg = upH;

@@ -243,6 +243,21 @@ uint32_t *gchunk_err(pa_ctx *c)
                         c->g_chunks);
 }
 
+// The trace store at `bytes`, or as large as the device allows: when the
+// allocation fails the store limit (trace_cap) halves, down to 64 MiB, and
+// the batch runs in more chunks.
+int grow_trace(pa_ctx *c, uint64_t bytes)
+{
+    for (;;) {
+        const int rc = grow(c, c->trace,
+                            std::min<uint64_t>(bytes, c->trace_cap),
+                            c->trace_cap);
+        if (rc != PA_ERR_NOMEM || c->trace_cap <= ((uint64_t)64 << 20))
+            return rc;
+        c->trace_cap /= 2;
+    }
+}
+
 // Runs the DP kernels (and the traceback when traced) on tasks [0, nt).
 int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
            hipStream_t s)
@@ -304,12 +319,14 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
         HIPCHK(c, hipMemcpyAsync(&total, P<uint64_t>(c->toffs) + nt, 8,
                                  hipMemcpyDeviceToHost, s));
         HIPCHK(c, hipStreamSynchronize(s));
-        if (total <= c->trace_cap) {
+        if (total <= c->trace_cap &&
+            (rc = grow_trace(c, std::max<uint64_t>(total, 64))) == PA_OK &&
+            c->trace.cap >= total) {
             chunks.push_back({0, nt});
-            if ((rc = grow(c, c->trace, std::max<uint64_t>(total, 64))))
-                return rc;
         } else {
-            if ((rc = grow(c, c->trace, c->trace_cap)))
+            if (rc && rc != PA_ERR_NOMEM)
+                return rc;
+            if ((rc = grow_trace(c, std::min<uint64_t>(c->trace_cap, total))))
                 return rc;
             // cut into chunks of at most trace_cap bytes on the device
             // (k_trace_chunks); only the ranges and bases come back
@@ -997,8 +1014,9 @@ int pa_create(const pa_config *cfg, pa_ctx **out)
     noalloc_init(c->device);
     size_t fr = 0, tot = 0;
     hipMemGetInfo(&fr, &tot);
-    // the trace store may take up to 40 % of free HBM (96 GB at most)
-    c->trace_cap = std::min<size_t>((size_t)96 << 30, (size_t)(fr * 0.4));
+    // the trace store may take up to 70 % of the HBM free now (it is
+    // allocated as batches need it, and halves if an allocation fails)
+    c->trace_cap = (size_t)(fr * 0.7);
     if (const char *e = std::getenv("PA_TRACE_BYTES"))
         c->trace_cap = std::max<size_t>(std::strtoull(e, nullptr, 10), 1 << 20);
     *out = c;
@@ -1369,12 +1387,14 @@ int pa_align_graph(pa_ctx *c, const pa_pair *d_pairs, uint64_t n,
         // 2. the captured layout: a store of twice this batch's trace
         //    (within the context's trace memory), split into as many chunks
         //    as this batch needs plus one
-        const uint64_t cap = std::max<uint64_t>(
+        uint64_t cap = std::max<uint64_t>(
             64, std::min<uint64_t>(c->trace_cap, 2 * total + (1u << 20)));
+        if ((rc = grow_trace(c, cap)))
+            return rc;
+        cap = std::min<uint64_t>(cap, c->trace_cap);   // if the store shrank
         const int chunks = (int)std::min<uint64_t>(
             64, (total + cap - 1) / cap + 1);
-        if ((rc = grow(c, c->trace, cap)) ||
-            (rc = grow(c, c->gchunk, (size_t)chunks * 16 + 16)))
+        if ((rc = grow(c, c->gchunk, (size_t)chunks * 16 + 16)))
             return rc;
         if (c->cfg.mode == PA_MODE_XDROP) {
             int rcap = 64;

@@ -2,6 +2,8 @@
 // the host-side sources of libpastis_hip.so.
 #pragma once
 
+#include <algorithm>
+#include <cstdint>
 #include <string>
 #include <vector>
 
@@ -168,7 +170,9 @@ inline int fail(pa_ctx *c, int code, const std::string &msg)
                         std::string(#x) + ": " + hipGetErrorString(e_));     \
     } while (0)
 
-inline int grow(pa_ctx *c, Buf &b, size_t bytes)
+// Buffer of at least `bytes`, allocated with a quarter of slack (at most up
+// to `limit`) so that slightly larger requests reuse it.
+inline int grow(pa_ctx *c, Buf &b, size_t bytes, size_t limit = SIZE_MAX)
 {
     if (b.cap >= bytes && b.p)
         return PA_OK;
@@ -178,7 +182,7 @@ inline int grow(pa_ctx *c, Buf &b, size_t bytes)
         hipFree(b.p);
     b.p = nullptr;
     b.cap = 0;
-    size_t want = bytes + bytes / 4 + 256;
+    size_t want = std::max(bytes, std::min(bytes + bytes / 4 + 256, limit));
     if (hipMalloc(&b.p, want) != hipSuccess) {
         b.p = nullptr;
         return fail(c, PA_ERR_NOMEM,
