@@ -499,12 +499,24 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
                     c->stats.launches += 3;
                 }
                 if (split) {
-                    // this class's trace is complete: trace it back on st2
+                    // this class's trace is complete: it is traced back on
+                    // st2, launched after the next class's DP kernels (in a
+                    // captured graph, whose kernels share one queue, the DP
+                    // grid is then dispatched ahead of the walks)
                     HIPCHK(c, hipEventRecord(c->ev_fork[rcl], s));
-                    HIPCHK(c, hipStreamWaitEvent(c->st2, c->ev_fork[rcl], 0));
-                    if ((rc = traceback(rcl, c->st2)))
-                        return rc;
+                    if (rcl > 0) {
+                        HIPCHK(c, hipStreamWaitEvent(c->st2,
+                                                     c->ev_fork[rcl - 1], 0));
+                        if ((rc = traceback(rcl - 1, c->st2)))
+                            return rc;
+                    }
                 }
+            }
+            if (split) {
+                HIPCHK(c, hipStreamWaitEvent(c->st2,
+                                             c->ev_fork[kClasses - 1], 0));
+                if ((rc = traceback(kClasses - 1, c->st2)))
+                    return rc;
             }
         }
         if (split) {
