@@ -243,11 +243,26 @@ uint32_t *gchunk_err(pa_ctx *c)
                         c->g_chunks);
 }
 
-// The trace store at `bytes`, or as large as the device allows: when the
-// allocation fails the store limit (trace_cap) halves, down to 64 MiB, and
-// the batch runs in more chunks.
+// The trace store at `bytes`, or as large as the device allows: the store
+// limit (trace_cap) is lowered to leave kTraceReserve of the device memory
+// free now for everything else, and halves when the allocation still fails
+// (down to 64 MiB); the batch then runs in more chunks.
+constexpr uint64_t kTraceReserve = (uint64_t)16 << 30;
+
 int grow_trace(pa_ctx *c, uint64_t bytes)
 {
+    if (c->trace.cap < std::min<uint64_t>(bytes, c->trace_cap)) {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) == hipSuccess) {
+            // the store's own memory comes back when it is reallocated
+            const uint64_t room = (uint64_t)fr + c->trace.cap;
+            const uint64_t limit =
+                room > kTraceReserve + ((uint64_t)64 << 20)
+                    ? room - kTraceReserve
+                    : (uint64_t)64 << 20;
+            c->trace_cap = std::min<uint64_t>(c->trace_cap, limit);
+        }
+    }
     for (;;) {
         const int rc = grow(c, c->trace,
                             std::min<uint64_t>(bytes, c->trace_cap),

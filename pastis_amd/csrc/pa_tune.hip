@@ -14,14 +14,16 @@
 //   5. traceback staging, by traceback time;
 //   6. X-drop items outgrowing the per-lane rings: the band kernels' window
 //      chain or global rings, by X-drop time.
-// A candidate replaces the incumbent only when it is faster by more than
-// kMinGain, so timing noise does not flip choices.  A sample can mislead on
+// A candidate replaces the incumbent only when that time is lower by more
+// than kMinGain, so timing noise does not flip choices, and the wall time
+// of the whole call is not higher by more than kMinGain.  A sample can mislead on
 // heavy-tailed length mixes (a few long tasks set a kernel's time), so when
 // the sample is smaller than the caller's set, the choice is checked against
 // the starting tuning on up to kValidate evenly spaced pairs of the whole
-// set and kept only if it is faster there too.  Results never depend on the
+// set and kept only if the whole call is faster there too.  Results never depend on the
 // tuning; only the speed does.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -43,7 +45,7 @@ constexpr int kReps = 2;
 
 struct Times {
     double dp = 0, cls[kClasses] = {0, 0, 0, 0}, trace = 0, xdrop = 0;
-    double total() const { return dp + trace + xdrop; }
+    double wall = 0;   // host time of the whole call, synchronised
 };
 
 std::mutex g_mu;
@@ -74,12 +76,19 @@ public:
         if ((rc = run()))
             return rc;
         Times b;
-        b.dp = b.trace = b.xdrop = INFINITY;
+        b.dp = b.trace = b.xdrop = b.wall = INFINITY;
         for (int k = 0; k < kClasses; ++k)
             b.cls[k] = INFINITY;
         for (int r = 0; r < kReps; ++r) {
-            if ((rc = pa_reset_stats(c_)) || (rc = run()))
+            if ((rc = pa_reset_stats(c_)))
                 return rc;
+            const auto t0 = std::chrono::steady_clock::now();
+            if ((rc = run()))
+                return rc;
+            b.wall = std::min(
+                b.wall, std::chrono::duration<double, std::milli>(
+                            std::chrono::steady_clock::now() - t0)
+                            .count());
             pa_stats st;
             if ((rc = pa_get_stats(c_, &st)))
                 return rc;
@@ -124,6 +133,14 @@ private:
 
 bool faster(double cand, double inc) { return cand < inc * (1.0 - kMinGain); }
 
+// A candidate wins when the stage it affects is faster and the whole call is
+// not slower (a shape can speed up its class's kernels and still lengthen
+// the call, through the trace store it needs or the walks it overlaps).
+bool better(double cand, double inc, const Times &c, const Times &i)
+{
+    return faster(cand, inc) && c.wall <= i.wall * (1.0 + kMinGain);
+}
+
 // The coordinate search; `t` starts as the installed tuning and ends as the
 // choice.
 int search(Tuner &tu, bool xdrop, pa_tuning &t)
@@ -148,7 +165,7 @@ int search(Tuner &tu, bool xdrop, pa_tuning &t)
                     Times tm;
                     if ((rc = tu.measure(cand, &tm)))
                         return rc;
-                    if (faster(tm.xdrop, cur.xdrop)) {
+                    if (better(tm.xdrop, cur.xdrop, tm, cur)) {
                         t = cand;
                         cur = tm;
                     }
@@ -167,7 +184,7 @@ int search(Tuner &tu, bool xdrop, pa_tuning &t)
         Times tm;
         if ((rc = tu.measure(cand, &tm)))
             return rc;
-        if (faster(tm.dp + tm.trace, cur.dp + cur.trace)) {
+        if (better(tm.dp + tm.trace, cur.dp + cur.trace, tm, cur)) {
             t = cand;
             cur = tm;
         }
@@ -183,7 +200,8 @@ int search(Tuner &tu, bool xdrop, pa_tuning &t)
             Times tm;
             if ((rc = tu.measure(cand, &tm)))
                 return rc;
-            if (faster(tm.cls[k] + tm.trace, cur.cls[k] + cur.trace)) {
+            if (better(tm.cls[k] + tm.trace, cur.cls[k] + cur.trace, tm,
+                       cur)) {
                 t = cand;
                 cur = tm;
             }
@@ -198,7 +216,7 @@ int search(Tuner &tu, bool xdrop, pa_tuning &t)
         Times tm;
         if ((rc = tu.measure(cand, &tm)))
             return rc;
-        if (faster(tm.dp, cur.dp)) {
+        if (better(tm.dp, cur.dp, tm, cur)) {
             t = cand;
             cur = tm;
         }
@@ -211,7 +229,7 @@ int search(Tuner &tu, bool xdrop, pa_tuning &t)
         Times tm;
         if ((rc = tu.measure(cand, &tm)))
             return rc;
-        if (faster(tm.trace, cur.trace)) {
+        if (better(tm.trace, cur.trace, tm, cur)) {
             t = cand;
             cur = tm;
         }
@@ -227,7 +245,7 @@ int search(Tuner &tu, bool xdrop, pa_tuning &t)
             Times tm;
             if ((rc = tu.measure(cand, &tm)))
                 return rc;
-            if (faster(tm.xdrop, cur.xdrop)) {
+            if (better(tm.xdrop, cur.xdrop, tm, cur)) {
                 t = cand;
                 cur = tm;
             }
@@ -306,7 +324,7 @@ extern "C" int pa_autotune(pa_ctx *c, const pa_pair *d_pairs, uint64_t n,
             Tuner tv(c, dp, mv, target, dout);
             Times a, b;
             if (!(rc = tv.measure(start, &a)) && !(rc = tv.measure(t, &b)) &&
-                !faster(b.total(), a.total()))
+                !faster(b.wall, a.wall))
                 t = start;
         }
     }
