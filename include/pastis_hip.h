@@ -428,7 +428,11 @@ typedef struct pa_kernel_stat {
 
 /* Kernel shapes of the DP kernels: L lanes per task (64 / L tasks per wave)
  * and R rows per lane per half-strip, so one pass covers 2 * R * L rows;
- * longer tasks take several passes. */
+ * longer tasks take several passes.  The R = 8 shapes keep 4-bit query
+ * profiles; with a scoring matrix whose scores over the alphabet span more
+ * than 16 values (or whose minimum is not negative) the context runs the
+ * R = 4 shape with the same rows per pass in their place (pa_get_tuning
+ * still returns the shapes that were set). */
 enum pa_dp_shape {
     PA_SHAPE_L16R4 = 0,     /* 128 rows per pass */
     PA_SHAPE_L16R8 = 1,     /* 256 */
