@@ -48,10 +48,12 @@ struct WaveArgs {
     uint32_t *ovf_list, *ovf_count;
 };
 
-// lane j of each 16-lane row reads lane j - K (mod 16) of its row
+// lane j of each 16-lane row reads lane j - K (mod 16) of its row (every
+// lane has a source, so no "old" operand: the compiler can fold the move
+// into the instruction that uses it)
 template <int K> __device__ inline int32_t ror(int32_t v)
 {
-    return __builtin_amdgcn_update_dpp(v, v, 0x120 + K, 0xf, 0xf, false);
+    return __builtin_amdgcn_mov_dpp(v, 0x120 + K, 0xf, 0xf, false);
 }
 
 // Column col - 1 of the lanes' columns of a chunk: lanes 1..15 take lane
@@ -72,6 +74,18 @@ __device__ inline int32_t row_max(int32_t v)
     v = o > v ? o : v;
     o = ror<8>(v);
     return o > v ? o : v;
+}
+
+__device__ inline int32_t row_min(int32_t v)
+{
+    int32_t o = ror<1>(v);
+    v = o < v ? o : v;
+    o = ror<2>(v);
+    v = o < v ? o : v;
+    o = ror<4>(v);
+    v = o < v ? o : v;
+    o = ror<8>(v);
+    return o < v ? o : v;
 }
 
 // maximum over the wave of a non-negative value
@@ -140,13 +154,20 @@ __global__ __launch_bounds__(64) void k_xdrop_band(WaveArgs a)
     int q[kChunks];             // query residue of the column
     int32_t sc[kChunks];        // substitution score of the next step's cell
     int B0[kChunks], B1[kChunks];   // database residues, loaded ahead
+    // column - 1 of A2 in this lane's columns, saved by each step: the next
+    // step's column - 1 of A1 (its diagonal input)
+    int32_t DL[kChunks];
 #pragma unroll
     for (int c = 0; c < kChunks; ++c) {
         A1[c] = A2[c] = A3[c] = kUndef;
         q[c] = 0;
         sc[c] = 0;
         B0[c] = B1[c] = 0;
+        DL[c] = kUndef;
     }
+    // DL does not hold this row's values (new item, or the window moved)
+    bool fresh = true;
+    int cw_prev = -1;   // chunks the previous step computed
     uint32_t cbase = 0, cend = 0;   // claimed, not yet handed out
     // highest chunk any running row reaches in the coming step (set at the
     // end of each step; a new item starts in chunk 0)
@@ -154,11 +175,11 @@ __global__ __launch_bounds__(64) void k_xdrop_band(WaveArgs a)
 
     auto qres = [&](int col) {   // query residue of column col (clamped)
         col = col < 1 ? 1 : (col > cols - 1 ? cols - 1 : col);
-        return (int)qb[qstep * col];
+        return (int)qb[qstep > 0 ? col : -col];
     };
     auto dres = [&](int row) {   // database residue of row (clamped)
         row = row < 1 ? 1 : (row > rows - 1 ? rows - 1 : row);
-        return (int)db[dstep * row];
+        return (int)db[dstep > 0 ? row : -row];
     };
     // group value of column x of anti-diagonal array A (x in the window)
     auto at = [&](const int32_t *A, int x) {
@@ -237,6 +258,7 @@ __global__ __launch_bounds__(64) void k_xdrop_band(WaveArgs a)
                         sc[c] = mat[q[c] * 32 + dres(2 - col)];
                         LOOK[c] = dres(3 - col);
                     }
+                    fresh = true;
                     state = kRun;
                 }
             }
@@ -282,49 +304,61 @@ __global__ __launch_bounds__(64) void k_xdrop_band(WaveArgs a)
         // the chunks any running row reads or writes this step: columns up
         // to its maxCol0 (the others hold columns no step reads)
         const int cw = cw_next;
-        // the new minCol and maxCol from ballots taken chunk by chunk:
-        // minCol = the first column c >= minCol that does not satisfy
-        // (A3[c] undefined, c - 1 in A2's range, A2[c-1] undefined), or
-        // maxCol0 + 1 when all of [minCol, maxCol0] do; maxCol = one past the
-        // last column x in [off3, maxCol0) with A3[x] or A2[x] defined (off3
-        // when none), plus one
-        int nmin = maxCol0 + 1, nmax = off3;
-        const int sh = g * kW;
+        // The diagonal input (column - 1 of A1) is the left input the
+        // previous step computed (column - 1 of its A2), unless a running
+        // row started an item or moved its window since, or the chunk is
+        // new this step.
+        const bool stale = __ballot(run && fresh) != 0;
+        // The new minCol is the first column c in [minCol, maxCol0] that
+        // does not satisfy (A3[c] undefined, c - 1 in A2's range, A2[c-1]
+        // undefined), or maxCol0 + 1 when all do; the new maxCol is one past
+        // the last column x in [off3, maxCol0) with A3[x] or A2[x] defined
+        // (off3 when none), plus one.  Each lane keeps its own candidates;
+        // one row minimum and maximum per step combine them.  x = col -
+        // minCol turns the ranges into single unsigned comparisons.
+        const int w = maxCol0 - minCol;
+        const int ecol0 = cond && off3 == 0 ? 0 : -1;   // edge columns
+        const int ecol1 = cond && d == maxCol0 ? maxCol0 : -1;
+        const int dlim = off2 + 1 + l2;   // c - 1 in A2's range: c < dlim
+        int32_t lo = INT_MAX, hi = INT_MIN;
 #pragma unroll
         for (int c = 0; c < kChunks; ++c) {
             if (c > cw)
                 break;
             const int col = base + kW * c + j;
+            const int x = col - minCol;
             const int32_t u = A2[c];
             const int32_t l = left_of(u, c ? A2[c - 1] : kUndef);
-            const int32_t dl = left_of(A1[c], c ? A1[c - 1] : kUndef);
+            int32_t dl;
+            if (stale || c > cw_prev)
+                dl = left_of(A1[c], c ? A1[c - 1] : kUndef);
+            else
+                dl = DL[c];
+            DL[c] = l;
             int32_t tmp = (l > u ? l : u) + gap;
             const int32_t dg = dl + sc[c];
             tmp = dg > tmp ? dg : tmp;
-            const bool inb = col >= minCol && col < maxCol0;
+            const bool inb = (uint32_t)x < (uint32_t)w;
             const int32_t val = tmp < lim ? kUndef : tmp;
-            const bool edge =
-                cond && ((col == 0 && off3 == 0) ||
-                         (col == maxCol0 && d == maxCol0));
+            const bool edge = col == ecol0 || col == ecol1;
             const int32_t a3 = inb ? val : (edge ? dgap : kUndef);
             A3[c] = a3;
             adbest = inb && val > adbest ? val : adbest;
-            const bool dead = a3 == kUndef && col - off2 - 1 < l2 && l == kUndef;
-            const uint32_t mlo =
-                (uint32_t)(__ballot(col >= minCol && col <= maxCol0 && !dead) >>
-                           sh) & 0xffffu;
-            const uint32_t mhi =
-                (uint32_t)(__ballot(col >= off3 && col < maxCol0 &&
-                                    !(a3 == kUndef && u == kUndef)) >> sh) &
-                0xffffu;
-            const int lo = base + kW * c + __builtin_ctz(mlo | 0x10000u);
-            const int hi = base + kW * c + (31 - __builtin_clz(mhi | 1u)) + 1;
-            nmin = mlo && lo < nmin ? lo : nmin;
-            nmax = mhi && hi > nmax ? hi : nmax;
+            const bool u3 = a3 == kUndef;
+            if ((uint32_t)x <= (uint32_t)w && !(u3 && col < dlim && l == kUndef))
+                lo = col < lo ? col : lo;
+            if ((uint32_t)(x + 1) < (uint32_t)(w + 1) && !(u3 && u == kUndef))
+                hi = col + 1;
         }
+        fresh = false;
+        cw_prev = cw;
         nc += (uint32_t)(maxCol0 - minCol);
         adbest = row_max(adbest);
         best = adbest > best ? adbest : best;
+        lo = row_min(lo);
+        hi = row_max(hi);
+        int nmin = lo == INT_MAX ? maxCol0 + 1 : lo;
+        int nmax = hi == INT_MIN ? off3 : hi;
         ++nmax;
         if (nmin < d + 2 - rows)
             nmin = d + 2 - rows;
@@ -411,6 +445,7 @@ __global__ __launch_bounds__(64) void k_xdrop_band(WaveArgs a)
                 LOOK[c] = LOOK[c + 1];
             }
             base += kW;
+            fresh = true;
             const int col = base + kW * (kChunks - 1) + j;
             A1[kChunks - 1] = A2[kChunks - 1] = A3[kChunks - 1] = kUndef;
             q[kChunks - 1] = qres(col);
@@ -419,9 +454,11 @@ __global__ __launch_bounds__(64) void k_xdrop_band(WaveArgs a)
         // scores of the next step from the residues loaded a half ago, and
         // the residues of the step after, for the chunks the next steps can
         // reach (the band's top moves up at most one column per step)
+        // (the next step computes chunks up to the reach, the one after at
+        // most one chunk more: its band's top is at most two columns higher)
         const int reach = wave_max(run ? (maxCol - base) / kW : 0);
         cw_next = min(kChunks - 1, reach);
-        const int pw = min(kChunks - 1, reach + 1);
+        const int pw = min(kChunks - 1, reach);
         if (run) {
 #pragma unroll
             for (int c = 0; c < kChunks; ++c) {

@@ -172,6 +172,13 @@ inline int emu_update_dpp(int old, int src, int ctrl, int, int, bool)
     std::abort();
 }
 #define __builtin_amdgcn_update_dpp emu_update_dpp
+// DPP move without an "old" operand (used for patterns where every lane has
+// a source)
+inline int emu_mov_dpp(int src, int ctrl, int rm, int bm, bool bc)
+{
+    return emu_update_dpp(0, src, ctrl, rm, bm, bc);
+}
+#define __builtin_amdgcn_mov_dpp emu_mov_dpp
 inline int max(int a, int b) { return a > b ? a : b; }
 inline int min(int a, int b) { return a < b ? a : b; }
 inline unsigned emu_readfirstlane(unsigned v)
