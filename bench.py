@@ -360,8 +360,9 @@ def measure_sw_config(args, which):
     default_ms = (time.perf_counter() - t0) * 1e3
     if not args.no_autotune:
         from pastis_amd import autotune
-        autotune.tune(al, (d_pairs, min(batch, n)), xdrop=False, force=True,
-                      log=log)
+        # evenly spaced samples of the whole set (the heavy tail of C3 is
+        # not spread evenly over the batches)
+        autotune.tune(al, (d_pairs, n), xdrop=False, force=True, log=log)
         run(False)
         al.synchronize()
     al.reset_stats()

@@ -457,10 +457,10 @@ typedef struct pa_tuning {
                                strictly increasing, 1..65535; default
                                128, 256, 512 */
     int32_t dp_shape[4];    /* pa_dp_shape of each row class; default
-                               L16R4, L16R8, L64R4, L64R8 */
-    int32_t tb_stage;       /* 1: the traceback copies each lane run of the
-                               trace it enters into LDS with wide loads;
-                               0: one byte load per step */
+                               L8R8, L16R8, L32R8, L64R8 */
+    int32_t tb_stage;       /* 1 (default): the traceback copies each lane
+                               run of the trace it enters into LDS with wide
+                               loads; 0: one byte load per step */
     int32_t xd_wave;        /* X-drop items whose band outgrows the
                                per-lane kernel's rings continue with 16
                                lanes per item: 1 (default) in a 128-column

@@ -452,9 +452,12 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
         tb.local = local ? 1 : 0;
         // traceback of one class's tasks: its two bin lists (bin = 2 * class
         // + arithmetic)
+        // Local tasks always run in int16 (saturated ones re-run in int32
+        // afterwards), so their int32 bins stay empty and are not launched.
+        const int nw = local ? 1 : 2;
         auto traceback = [&](int cls, hipStream_t ts) -> int {
             Scope sc(c, 3, ts);
-            for (int w = 0; w < 2; ++w) {
+            for (int w = 0; w < nw; ++w) {
                 const int bin = 2 * cls + w;
                 tb.list = lists + (size_t)bin * nt;
                 tb.count = ctr + C_BIN + bin;
@@ -474,7 +477,7 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
                 args.pmat = rows_nib(shape_rows(c->shapes.shape[rcl]))
                                 ? pmat_nib
                                 : pmat_byte;
-                for (int w = 0; w < 2; ++w) {
+                for (int w = 0; w < nw; ++w) {
                     const int bin = 2 * rcl + w;
                     args.list = lists + (size_t)bin * nt;
                     args.count = ctr + C_BIN + bin;

@@ -40,7 +40,7 @@ inline pa_tuning default_tuning()
         t.dp_edge[c] = (int32_t)kDefaultShapes.edge[c];
     for (int c = 0; c < 4; ++c)
         t.dp_shape[c] = kDefaultShapes.shape[c];
-    t.tb_stage = 0;
+    t.tb_stage = 1;
     t.xd_wave = 1;
     return t;
 }

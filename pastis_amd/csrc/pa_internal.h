@@ -178,7 +178,7 @@ struct DpShapes {
     uint8_t shape[4];
 };
 constexpr DpShapes kDefaultShapes = {{128, 256, 512},
-                                     {kShapeL16R4, kShapeL16R8, kShapeL64R4,
+                                     {kShapeL8R8, kShapeL16R8, kShapeL32R8,
                                       kShapeL64R8}};
 
 __host__ __device__ inline int dp_class(uint32_t qlen, const DpShapes &sh)

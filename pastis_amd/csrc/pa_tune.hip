@@ -40,7 +40,7 @@ namespace {
 
 constexpr double kMinGain = 0.01;
 constexpr uint64_t kDefaultSample = 262144;
-constexpr uint64_t kValidate = 1u << 22;
+constexpr uint64_t kValidate = 1u << 23;
 constexpr int kReps = 2;
 
 struct Times {
