@@ -96,7 +96,8 @@ class PaTuning(C.Structure):
     _fields_ = [("dp_wg_per_cu", C.c_int32), ("xd_wg_per_cu", C.c_int32),
                 ("xd_ring", C.c_int32), ("xd_coop", C.c_int32),
                 ("dp_edge", C.c_int32 * 3), ("dp_shape", C.c_int32 * 4),
-                ("tb_stage", C.c_int32), ("xd_wave", C.c_int32)]
+                ("tb_stage", C.c_int32), ("xd_wave", C.c_int32),
+                ("tb_overlap", C.c_int32)]
 
     def as_dict(self):
         """Scalars as ints, dp_edge / dp_shape as tuples."""

@@ -395,7 +395,8 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
     // tracebacks on the second stream, overlapping later classes' DP
     // (also inside a pa_align_graph capture: the event fork and join
     // bring the second stream into the graph)
-    const bool split = trace && c->st2 && (!c->capturing || c->g_chunks > 0);
+    const bool split = trace && c->st2 && c->tune.tb_overlap != 0 &&
+                       (!c->capturing || c->g_chunks > 0);
     const bool dev_chunks = trace && c->g_chunks > 0;
     for (size_t ci = 0; ci < chunks.size(); ++ci) {
         const auto &ch = chunks[ci];
@@ -470,14 +471,15 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
         };
         {
             Scope sc(c, kind, s);
-            for (int rcl = 0; rcl < kClasses; ++rcl) {
+            // DP of class rcl's bin w (0: int16, 1: int32)
+            auto dp_bin = [&](int rcl, int w) -> int {
                 // short tasks are claimed many per atomic, long ones singly
                 // (the 16-lane classes claim rounds of four tasks)
                 args.chunk = rcl <= 2 ? 4u : 1u;
                 args.pmat = rows_nib(shape_rows(c->shapes.shape[rcl]))
                                 ? pmat_nib
                                 : pmat_byte;
-                for (int w = 0; w < nw; ++w) {
+                {
                     const int bin = 2 * rcl + w;
                     args.list = lists + (size_t)bin * nt;
                     args.count = ctr + C_BIN + bin;
@@ -494,6 +496,19 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
                     }
                     c->stats.launches++;
                 }
+                return PA_OK;
+            };
+            // The int32 bins (global tasks that may leave the int16 range,
+            // usually none) of every class first, while the GPU is free: a
+            // launch queued between the classes waits for the CUs that the
+            // walks of the previous class hold, even when its bin is empty.
+            if (nw == 2)
+                for (int rcl = 0; rcl < kClasses; ++rcl)
+                    if ((rc = dp_bin(rcl, 1)))
+                        return rc;
+            for (int rcl = 0; rcl < kClasses; ++rcl) {
+                if ((rc = dp_bin(rcl, 0)))
+                    return rc;
                 if (local) {
                     // exact int32 re-run of the tasks whose int16 DP saturated
                     args.list = sat + (size_t)rcl * nt;
@@ -543,9 +558,24 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
             HIPCHK(c, hipEventRecord(c->ev_join, c->st2));
             HIPCHK(c, hipStreamWaitEvent(s, c->ev_join, 0));
         } else if (trace) {
-            for (int cls = 0; cls < kClasses; ++cls)
+            // after all the DP: the few long walks of the last class on the
+            // second stream, beside the many short ones of the others
+            const bool two =
+                c->st2 && (!c->capturing || c->g_chunks > 0);
+            if (two) {
+                HIPCHK(c, hipEventRecord(c->ev_fork[kClasses - 1], s));
+                HIPCHK(c, hipStreamWaitEvent(c->st2,
+                                             c->ev_fork[kClasses - 1], 0));
+                if ((rc = traceback(kClasses - 1, c->st2)))
+                    return rc;
+            }
+            for (int cls = 0; cls < (two ? kClasses - 1 : kClasses); ++cls)
                 if ((rc = traceback(cls, s)))
                     return rc;
+            if (two) {
+                HIPCHK(c, hipEventRecord(c->ev_join, c->st2));
+                HIPCHK(c, hipStreamWaitEvent(s, c->ev_join, 0));
+            }
         }
     }
     return PA_OK;
@@ -1113,6 +1143,8 @@ int pa_set_tuning(pa_ctx *c, const pa_tuning *t)
         return fail(c, PA_ERR_ARG, "tb_stage must be 0 or 1");
     if (t->xd_wave < 0 || t->xd_wave > 2)
         return fail(c, PA_ERR_ARG, "xd_wave must be 0, 1 or 2");
+    if (t->tb_overlap < 0 || t->tb_overlap > 1)
+        return fail(c, PA_ERR_ARG, "tb_overlap must be 0 or 1");
     for (int k = 0; k < 3; ++k)
         if (t->dp_edge[k] < 1 || t->dp_edge[k] > 65535 ||
             (k > 0 && t->dp_edge[k] <= t->dp_edge[k - 1]))

@@ -42,6 +42,7 @@ inline pa_tuning default_tuning()
         t.dp_shape[c] = kDefaultShapes.shape[c];
     t.tb_stage = 1;
     t.xd_wave = 1;
+    t.tb_overlap = 0;
     return t;
 }
 inline DpShapes shapes_of(const pa_tuning &t)

@@ -12,7 +12,9 @@
 //      traceback time (a shape also fixes the trace layout);
 //   4. persistent DP workgroups per CU, by DP time;
 //   5. traceback staging, by traceback time;
-//   6. X-drop items outgrowing the per-lane rings: the band kernels' window
+//   6. tracebacks beside the next classes' DP or after all of it, by the
+//      wall time of the call;
+//   7. X-drop items outgrowing the per-lane rings: the band kernels' window
 //      chain or global rings, by X-drop time.
 // A candidate replaces the incumbent only when that time is lower by more
 // than kMinGain, so timing noise does not flip choices, and the wall time
@@ -235,7 +237,20 @@ int search(Tuner &tu, bool xdrop, pa_tuning &t)
         }
     }
 
-    // 6. wide X-drop bands
+    // 6. tracebacks beside the DP or after it, by wall time
+    if (cur.trace > 0) {
+        pa_tuning cand = t;
+        cand.tb_overlap = 1 - t.tb_overlap;
+        Times tm;
+        if ((rc = tu.measure(cand, &tm)))
+            return rc;
+        if (faster(tm.wall, cur.wall)) {
+            t = cand;
+            cur = tm;
+        }
+    }
+
+    // 7. wide X-drop bands
     if (xdrop) {
         for (int wv = 0; wv < 3; ++wv) {
             if (wv == t.xd_wave)
