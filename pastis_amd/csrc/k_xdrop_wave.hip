@@ -155,15 +155,18 @@ __global__ __launch_bounds__(64) void k_xdrop_band(WaveArgs a)
     int32_t sc[kChunks];        // substitution score of the next step's cell
     int B0[kChunks], B1[kChunks];   // database residues, loaded ahead
     // column - 1 of A2 in this lane's columns, saved by each step: the next
-    // step's column - 1 of A1 (its diagonal input)
-    int32_t DL[kChunks];
+    // step's column - 1 of A1 (its diagonal input); kept by the 128-column
+    // pass only (the 256-column pass is at its register limit)
+    constexpr bool kKeepDL = NCH <= 8;
+    int32_t DL[kKeepDL ? kChunks : 1];
 #pragma unroll
     for (int c = 0; c < kChunks; ++c) {
         A1[c] = A2[c] = A3[c] = kUndef;
         q[c] = 0;
         sc[c] = 0;
         B0[c] = B1[c] = 0;
-        DL[c] = kUndef;
+        if (kKeepDL)
+            DL[c] = kUndef;
     }
     // DL does not hold this row's values (new item, or the window moved)
     bool fresh = true;
@@ -330,11 +333,12 @@ __global__ __launch_bounds__(64) void k_xdrop_band(WaveArgs a)
             const int32_t u = A2[c];
             const int32_t l = left_of(u, c ? A2[c - 1] : kUndef);
             int32_t dl;
-            if (stale || c > cw_prev)
+            if (!kKeepDL || stale || c > cw_prev)
                 dl = left_of(A1[c], c ? A1[c - 1] : kUndef);
             else
                 dl = DL[c];
-            DL[c] = l;
+            if constexpr (kKeepDL)
+                DL[c] = l;
             int32_t tmp = (l > u ? l : u) + gap;
             const int32_t dg = dl + sc[c];
             tmp = dg > tmp ? dg : tmp;
