@@ -469,8 +469,10 @@ typedef struct pa_tuning {
                                left runs with global-memory rings, one lane
                                per item */
     int32_t tb_overlap;     /* 1: each class's tracebacks run on a second
-                               stream beside the next classes' DP; 0
-                               (default): all tracebacks after the DP */
+                               stream beside the next classes' DP; 2: those
+                               of all classes but the last beside the last
+                               class's DP; 0 (default): all tracebacks
+                               after the DP */
 } pa_tuning;
 
 int pa_get_tuning(struct pa_ctx *ctx, pa_tuning *out);

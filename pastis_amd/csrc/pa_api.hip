@@ -395,8 +395,14 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
     // tracebacks on the second stream, overlapping later classes' DP
     // (also inside a pa_align_graph capture: the event fork and join
     // bring the second stream into the graph)
-    const bool split = trace && c->st2 && c->tune.tb_overlap != 0 &&
-                       (!c->capturing || c->g_chunks > 0);
+    // the second stream (also inside a pa_align_graph capture: the event
+    // fork and join bring it into the graph)
+    const bool two = c->st2 && (!c->capturing || c->g_chunks > 0);
+    // walks of each class beside the next classes' DP (tb_overlap 1), or
+    // of all classes but the last beside the last class's DP (2), or all
+    // after the DP (0)
+    const bool split = trace && two && c->tune.tb_overlap == 1;
+    const bool beside = trace && two && c->tune.tb_overlap == 2;
     const bool dev_chunks = trace && c->g_chunks > 0;
     for (size_t ci = 0; ci < chunks.size(); ++ci) {
         const auto &ch = chunks[ci];
@@ -544,6 +550,17 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
                             return rc;
                     }
                 }
+                if (beside && rcl == kClasses - 2)
+                    HIPCHK(c, hipEventRecord(c->ev_fork[rcl], s));
+            }
+            if (beside) {
+                // launched after the last class's DP kernels (so that a
+                // captured graph dispatches that grid first)
+                HIPCHK(c, hipStreamWaitEvent(c->st2,
+                                             c->ev_fork[kClasses - 2], 0));
+                for (int cls = 0; cls < kClasses - 1; ++cls)
+                    if ((rc = traceback(cls, c->st2)))
+                        return rc;
             }
             if (split) {
                 HIPCHK(c, hipStreamWaitEvent(c->st2,
@@ -557,11 +574,15 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
             // the caller reads the traceback results, all on s
             HIPCHK(c, hipEventRecord(c->ev_join, c->st2));
             HIPCHK(c, hipStreamWaitEvent(s, c->ev_join, 0));
+        } else if (beside) {
+            // the last class's walks after its DP; the others' join
+            if ((rc = traceback(kClasses - 1, s)))
+                return rc;
+            HIPCHK(c, hipEventRecord(c->ev_join, c->st2));
+            HIPCHK(c, hipStreamWaitEvent(s, c->ev_join, 0));
         } else if (trace) {
             // after all the DP: the few long walks of the last class on the
             // second stream, beside the many short ones of the others
-            const bool two =
-                c->st2 && (!c->capturing || c->g_chunks > 0);
             if (two) {
                 HIPCHK(c, hipEventRecord(c->ev_fork[kClasses - 1], s));
                 HIPCHK(c, hipStreamWaitEvent(c->st2,
@@ -1143,8 +1164,8 @@ int pa_set_tuning(pa_ctx *c, const pa_tuning *t)
         return fail(c, PA_ERR_ARG, "tb_stage must be 0 or 1");
     if (t->xd_wave < 0 || t->xd_wave > 2)
         return fail(c, PA_ERR_ARG, "xd_wave must be 0, 1 or 2");
-    if (t->tb_overlap < 0 || t->tb_overlap > 1)
-        return fail(c, PA_ERR_ARG, "tb_overlap must be 0 or 1");
+    if (t->tb_overlap < 0 || t->tb_overlap > 2)
+        return fail(c, PA_ERR_ARG, "tb_overlap must be 0, 1 or 2");
     for (int k = 0; k < 3; ++k)
         if (t->dp_edge[k] < 1 || t->dp_edge[k] > 65535 ||
             (k > 0 && t->dp_edge[k] <= t->dp_edge[k - 1]))

@@ -12,8 +12,8 @@
 //      traceback time (a shape also fixes the trace layout);
 //   4. persistent DP workgroups per CU, by DP time;
 //   5. traceback staging, by traceback time;
-//   6. tracebacks beside the next classes' DP or after all of it, by the
-//      wall time of the call;
+//   6. tracebacks beside the next classes' DP, beside the last class's DP
+//      or after all of it, by the wall time of the call;
 //   7. X-drop items outgrowing the per-lane rings: the band kernels' window
 //      chain or global rings, by X-drop time.
 // A candidate replaces the incumbent only when that time is lower by more
@@ -239,14 +239,19 @@ int search(Tuner &tu, bool xdrop, pa_tuning &t)
 
     // 6. tracebacks beside the DP or after it, by wall time
     if (cur.trace > 0) {
-        pa_tuning cand = t;
-        cand.tb_overlap = 1 - t.tb_overlap;
-        Times tm;
-        if ((rc = tu.measure(cand, &tm)))
-            return rc;
-        if (faster(tm.wall, cur.wall)) {
-            t = cand;
-            cur = tm;
+        const int start_ov = t.tb_overlap;
+        for (int ov = 0; ov < 3; ++ov) {
+            if (ov == start_ov)
+                continue;
+            pa_tuning cand = t;
+            cand.tb_overlap = ov;
+            Times tm;
+            if ((rc = tu.measure(cand, &tm)))
+                return rc;
+            if (faster(tm.wall, cur.wall)) {
+                t = cand;
+                cur = tm;
+            }
         }
     }
 

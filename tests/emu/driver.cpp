@@ -112,18 +112,22 @@ int main(int argc, char **argv)
     pa_tuning t;
     pa_get_tuning(c, &t);
     t.xd_coop = h[7];
-    // EMU_SHAPES = "s0,s1,s2,s3,e0,e1,e2,stage": row-class kernel shapes,
-    // class edges and traceback staging
+    // EMU_SHAPES = "s0,s1,s2,s3,e0,e1,e2,stage[,overlap]": row-class kernel
+    // shapes, class edges, traceback staging and traceback scheduling
     if (const char *e = std::getenv("EMU_SHAPES")) {
-        int v[8];
-        if (std::sscanf(e, "%d,%d,%d,%d,%d,%d,%d,%d", &v[0], &v[1], &v[2],
-                        &v[3], &v[4], &v[5], &v[6], &v[7]) != 8)
+        int v[9];
+        const int n = std::sscanf(e, "%d,%d,%d,%d,%d,%d,%d,%d,%d", &v[0],
+                                  &v[1], &v[2], &v[3], &v[4], &v[5], &v[6],
+                                  &v[7], &v[8]);
+        if (n != 8 && n != 9)
             return 2;
         for (int k = 0; k < 4; ++k)
             t.dp_shape[k] = v[k];
         for (int k = 0; k < 3; ++k)
             t.dp_edge[k] = v[4 + k];
         t.tb_stage = v[7];
+        if (n == 9)
+            t.tb_overlap = v[8];
     }
     if (const char *e = std::getenv("EMU_XDWAVE"))
         t.xd_wave = std::atoi(e);

@@ -71,9 +71,12 @@ def run(driver, tmp_path, w, pairs, mode, xdrop=0, coop=0,
 
 # Non-default launch shapes: 64-lane kernels on short rows and 16-lane
 # multi-pass kernels (128 rows per pass) on long ones, class edges that send
-# most tasks through several passes, LDS-staged tracebacks; then the 32- and
-# 8-lane kernels (eight and two tasks per wave) with multi-pass tasks.
-SHAPES = [None, (3, 2, 1, 0, 16, 40, 100, 1), (4, 5, 4, 5, 16, 40, 100, 1)]
+# most tasks through several passes, LDS-staged tracebacks, each class's
+# walks beside the next classes' DP; then the 32- and 8-lane kernels (eight
+# and two tasks per wave) with multi-pass tasks and the walks beside the last
+# class's DP.
+SHAPES = [None, (3, 2, 1, 0, 16, 40, 100, 1, 1),
+          (4, 5, 4, 5, 16, 40, 100, 1, 2)]
 
 
 @pytest.mark.parametrize("shapes", SHAPES)

@@ -18,18 +18,19 @@ from pastis_amd.aligner import Aligner, config_for_cli
 
 pytestmark = pytest.mark.gpu
 
-# (dp_shape per class, dp_edge, tb_stage); shapes 0..5 = l16r4, l16r8,
-# l64r4, l64r8, l32r8, l8r8
+# (dp_shape per class, dp_edge, tb_stage, tb_overlap); shapes 0..5 = l16r4,
+# l16r8, l64r4, l64r8, l32r8, l8r8
 LAYOUTS = [
-    ((5, 1, 4, 3), (128, 256, 512), 1),      # the defaults
-    ((0, 1, 2, 3), (128, 256, 512), 0),      # 4-row shapes on short rows
-    ((0, 0, 0, 0), (16, 40, 100), 1),        # 128-row passes everywhere
-    ((3, 3, 3, 3), (128, 256, 512), 0),      # 1024-row passes everywhere
-    ((3, 2, 1, 0), (16, 40, 100), 1),        # wide on short, narrow on long
-    ((1, 0, 3, 2), (200, 300, 1000), 1),
-    ((5, 1, 4, 3), (128, 256, 512), 0),      # the defaults, walks unstaged
-    ((4, 5, 4, 5), (16, 40, 100), 1),        # 32- and 8-lane multi-pass tasks
-    ((5, 4, 5, 4), (16, 40, 100), 0),
+    ((5, 1, 4, 3), (128, 256, 512), 1, 0),   # the defaults
+    ((0, 1, 2, 3), (128, 256, 512), 0, 1),   # 4-row shapes on short rows
+    ((0, 0, 0, 0), (16, 40, 100), 1, 2),     # 128-row passes everywhere
+    ((3, 3, 3, 3), (128, 256, 512), 0, 0),   # 1024-row passes everywhere
+    ((3, 2, 1, 0), (16, 40, 100), 1, 1),     # wide on short, narrow on long
+    ((1, 0, 3, 2), (200, 300, 1000), 1, 0),
+    ((5, 1, 4, 3), (128, 256, 512), 0, 2),   # the defaults, walks unstaged
+    ((4, 5, 4, 5), (16, 40, 100), 1, 1),     # 32- and 8-lane multi-pass tasks
+    ((5, 4, 5, 4), (16, 40, 100), 0, 2),
+    ((5, 1, 4, 3), (128, 256, 512), 1, 2),   # walks beside the last DP
 ]
 
 
@@ -54,10 +55,11 @@ def _with_long(w, seed):
 
 
 def _set(al, layout):
-    sh, ed, st = layout
-    al.set_tuning(dp_shape=sh, dp_edge=ed, tb_stage=st)
+    sh, ed, st, ov = layout
+    al.set_tuning(dp_shape=sh, dp_edge=ed, tb_stage=st, tb_overlap=ov)
     t = al.tuning()
-    assert t["dp_shape"] == sh and t["dp_edge"] == ed and t["tb_stage"] == st
+    assert t["dp_shape"] == sh and t["dp_edge"] == ed and \
+        t["tb_stage"] == st and t["tb_overlap"] == ov
 
 
 def test_sw_layouts_match_the_oracle():
@@ -183,7 +185,7 @@ def test_bad_layouts_are_rejected():
     al = Aligner(config_for_cli(-11, -2, 6, 49, 2, "as_called"))
     for kw in [dict(dp_edge=(128, 128, 512)), dict(dp_edge=(0, 10, 20)),
                dict(dp_edge=(1, 2, 70000)), dict(dp_shape=(0, 1, 2, 6)),
-               dict(tb_stage=2)]:
+               dict(tb_stage=2), dict(tb_overlap=3), dict(tb_overlap=-1)]:
         with pytest.raises(_lib.PaError):
             al.set_tuning(**kw)
     al.close()
