@@ -12,15 +12,23 @@
 // by absolute column (slot = column mod ring size).  The algorithm only reads
 // columns inside the range written on that anti-diagonal ([off, maxCol]), so
 // the ring is exact while that range fits; a wider range reports overflow.
-//   - k_xdrop_pl<RING>: RING-slot rings in LDS ([3][RING][64 lanes] int32,
-//     RING = 16 or 32);
-//   - k_xdrop_pl<0>: rings in global scratch sized to the longest sequence
-//     (the overflow path).
+//   - k_xdrop_pl<RING>: RING-slot rings in LDS ([3][RING][64 lanes] int16,
+//     RING = 16 or 32).  A cell of anti-diagonal d is kept only if it is at
+//     least lim_d = best - X (best before d), and no cell exceeds best plus
+//     the largest matrix score, so a ring holds v - lim_d, in [0, X + 128]
+//     for int8 scores and gap <= 0, and -32768 for an undefined cell.  Each
+//     anti-diagonal keeps its own lim_d (its "frame"); the cell arithmetic
+//     runs in the frame of anti-diagonal d-1.  Items of calls with X above
+//     30000 or gaps outside [-1000, 0] overflow at once to the int32 kernels;
+//   - k_xdrop_pl<0>: int32 rings in global scratch sized to the longest
+//     sequence (the overflow path), frames all zero.
 // Scheduling: each lane advances its own item one anti-diagonal per loop
 // trip and takes the next item from the queue as soon as its item ends (one
 // atomic per wave for all lanes that need an item), so a long extension
 // occupies one lane, not a whole wave.
 #include <limits.h>
+
+#include <type_traits>
 
 #include "pa_internal.h"
 
@@ -28,16 +36,29 @@ namespace pa {
 namespace {
 
 constexpr int32_t kUndef = INT_MIN / 4;
+// int16 rings: the X-drop and gap values the cell encoding covers
+constexpr int kX16Max = 30000;
+constexpr int kGap16Min = -1000;
 
-struct Rings {
-    int32_t *base;
+template <class T> struct Rings {
+    T *base;
     int ring_stride;   // elements between the three rings
     int slot_stride;   // elements between consecutive slots
     int mask;          // slots - 1
     int slots;
-    __device__ inline int32_t &at(int r, int col) const
+    // stored value of an undefined cell
+    static constexpr int32_t kSent = sizeof(T) == 2 ? INT16_MIN : kUndef;
+    // values are kept relative to their anti-diagonal's frame
+    static constexpr bool kRel = sizeof(T) == 2;
+    __device__ inline T &at(int r, int col) const
     {
         return base[r * ring_stride + (col & mask) * slot_stride];
+    }
+    // the value of a cell (kUndef if undefined) from its stored form
+    __device__ inline int32_t get(int r, int col, int32_t frame) const
+    {
+        const int32_t v = at(r, col);
+        return v == kSent ? kUndef : v + frame;
     }
 };
 
@@ -49,6 +70,7 @@ struct Item {
     int d, minCol, maxCol, best;
     int off1, off2, off3, l1, l2, l3;
     int r1, r2, r3;           // ring of anti-diagonals d-2, d-1, d
+    int32_t f1, f2, f3;       // their frames (stored value = value - frame)
     uint32_t nc;              // evaluated cells
     // residue staging (LDS kernels): the request issued at the start of the
     // previous anti-diagonal is stored at the start of this one
@@ -68,12 +90,25 @@ __device__ inline int d_at(const Item &s, int row)
     return s.db[s.dstep * row];
 }
 
-// Per-lane LDS rings of residues, 32 slots each, indexed by column and row:
-// the band never spans more than 31 columns / rows ahead of what is staged.
+// Per-lane LDS rings of residues, as many slots as the cell rings (RING),
+// indexed by column and row.  An anti-diagonal that runs has maxCol - minCol
+// <= RING - 2, minCol never decreases, maxCol grows by at most one per step
+// and d - maxCol never decreases.  So the column stored at the start of step
+// d (maxCol of step d-1, <= minCol(d) + RING - 2) never lands on the slot of
+// a column >= minCol(d) still to be read, and the row stored then (d -
+// minCol(d-1)) is less than RING above the lowest row read from then on
+// (d - maxCol(d) + 1, with maxCol(d) - minCol(d-1) <= RING - 1).
 struct ResRings {
     uint8_t *q, *d;   // slot stride 64 (one byte per lane)
-    __device__ inline uint8_t &qs(int col) const { return q[(col & 31) * 64]; }
-    __device__ inline uint8_t &ds(int row) const { return d[(row & 31) * 64]; }
+    int mask;         // slots - 1
+    __device__ inline uint8_t &qs(int col) const
+    {
+        return q[(col & mask) * 64];
+    }
+    __device__ inline uint8_t &ds(int row) const
+    {
+        return d[(row & mask) * 64];
+    }
 };
 
 // Stores the request in flight into the rings, then requests column `col`
@@ -95,9 +130,9 @@ __device__ inline void stage(Item &s, const ResRings &Rr, int col, int row)
     s.rhi = dv ? row : s.rhi;
 }
 
-template <bool STAGE>
-__device__ inline void item_init(Item &s, const Rings &A, const ResRings &Rr,
-                                 int gap, int X)
+template <bool STAGE, class T>
+__device__ inline void item_init(Item &s, const Rings<T> &A,
+                                 const ResRings &Rr, int gap, int X)
 {
     s.r1 = 0;
     s.r2 = 1;
@@ -105,9 +140,13 @@ __device__ inline void item_init(Item &s, const Rings &A, const ResRings &Rr,
     s.l1 = 0;
     s.l2 = 1;
     s.l3 = 2;
-    A.at(s.r2, 0) = 0;
-    A.at(s.r3, 0) = -gap > X ? kUndef : gap;
-    A.at(s.r3, 1) = -gap > X ? kUndef : gap;
+    // anti-diagonals 0 and 1 in the frame of the first limit (best 0):
+    // frames never decrease
+    const int32_t f0 = A.kRel ? -X : 0;
+    s.f1 = s.f2 = s.f3 = f0;
+    A.at(s.r2, 0) = 0 - f0;
+    A.at(s.r3, 0) = -gap > X ? A.kSent : gap - f0;
+    A.at(s.r3, 1) = -gap > X ? A.kSent : gap - f0;
     s.minCol = 1;
     s.maxCol = 2;
     s.off1 = s.off2 = s.off3 = 0;
@@ -126,9 +165,10 @@ __device__ inline void item_init(Item &s, const Rings &A, const ResRings &Rr,
 
 // One anti-diagonal.  Returns 0 while the band is not empty, 1 when the
 // extension ended, 2 when the written range outgrew the rings.
-template <bool STAGE>
-__device__ inline int item_step(Item &s, const Rings &A, const ResRings &Rr,
-                                const int8_t *mat, int gap, int X)
+template <bool STAGE, class T>
+__device__ inline int item_step(Item &s, const Rings<T> &A,
+                                const ResRings &Rr, const int8_t *mat,
+                                int gap, int X)
 {
     // anti-diagonal d+1 (the next one) reads columns < maxCol(d+1) <=
     // maxCol + 1 and rows <= d + 1 - minCol(d+1) <= d + 1 - minCol
@@ -145,22 +185,32 @@ __device__ inline int item_step(Item &s, const Rings &A, const ResRings &Rr,
     s.off2 = s.off3;
     s.off3 = s.minCol - 1;
     s.l3 = s.maxCol + 1 - s.off3;
+    const int32_t lim = s.best - X;
+    s.f1 = s.f2;
+    s.f2 = s.f3;
+    s.f3 = A.kRel ? lim : 0;
     if (s.l3 > A.slots)
         return 2;
     const int r1 = s.r1, r2 = s.r2, r3 = s.r3;
     const int off3 = s.off3, maxCol0 = s.maxCol;
     // _initAntiDiag3: edge entries, first column / first row cells
-    A.at(r3, off3) = kUndef;
-    A.at(r3, maxCol0) = kUndef;
+    A.at(r3, off3) = A.kSent;
+    A.at(r3, maxCol0) = A.kSent;
     if ((long)d * gap > (long)s.best - X) {
         if (off3 == 0)
-            A.at(r3, off3) = d * gap;
+            A.at(r3, off3) = d * gap - s.f3;
         if (d - maxCol0 == 0)
-            A.at(r3, maxCol0) = d * gap;
+            A.at(r3, maxCol0) = d * gap - s.f3;
     }
-    int32_t adbest = d * gap;
-    const int32_t lim = s.best - X;
     const int cols = s.cols, rows = s.rows;
+    // the cells are computed in the frame of anti-diagonal d-1 (r2): c12
+    // moves an r1 value into it, lim2 is the limit and the frame of r3
+    // there.  An undefined stored value (kSent) stays below every defined
+    // candidate and below the limit after these shifts.
+    const int32_t c12 = s.f1 - s.f2;
+    const int32_t lim2 = lim - s.f2;
+    const int32_t st2 = s.f3 - s.f2;
+    int32_t adbest = d * gap - s.f2;
     // the up value of one cell is the left value of the next (r2 is not
     // written in this loop)
     int32_t l = A.at(r2, s.minCol - 1);
@@ -177,24 +227,25 @@ __device__ inline int item_step(Item &s, const Rings &A, const ResRings &Rr,
         int32_t tmp = (l > u ? l : u) + gap;
         const int qr = STAGE ? (int)Rr.qs(col) : (int)s.qs[qp];
         const int dr = STAGE ? (int)Rr.ds(d - col) : (int)s.ds[dp];
-        const int32_t dg = A.at(r1, col - 1) + mat[qr * 32 + dr];
+        const int32_t dg = A.at(r1, col - 1) + c12 + mat[qr * 32 + dr];
         tmp = dg > tmp ? dg : tmp;
-        if (tmp < lim) {
-            A.at(r3, col) = kUndef;
+        if (tmp < lim2) {
+            A.at(r3, col) = A.kSent;
         } else {
-            A.at(r3, col) = tmp;
+            A.at(r3, col) = tmp - st2;
             adbest = tmp > adbest ? tmp : adbest;
         }
         l = u;
     }
+    adbest += s.f2;
     s.nc += (uint32_t)(maxCol0 - s.minCol);
     s.best = adbest > s.best ? adbest : s.best;
     int minCol = s.minCol, maxCol = maxCol0;
-    while (minCol - off3 < s.l3 && A.at(r3, minCol) == kUndef &&
-           minCol - s.off2 - 1 < s.l2 && A.at(r2, minCol - 1) == kUndef)
+    while (minCol - off3 < s.l3 && A.at(r3, minCol) == A.kSent &&
+           minCol - s.off2 - 1 < s.l2 && A.at(r2, minCol - 1) == A.kSent)
         ++minCol;
-    while (maxCol - off3 > 0 && A.at(r3, maxCol - 1) == kUndef &&
-           A.at(r2, maxCol - 1) == kUndef)
+    while (maxCol - off3 > 0 && A.at(r3, maxCol - 1) == A.kSent &&
+           A.at(r2, maxCol - 1) == A.kSent)
         --maxCol;
     ++maxCol;
     if (minCol < d + 2 - rows)
@@ -208,27 +259,29 @@ __device__ inline int item_step(Item &s, const Rings &A, const ResRings &Rr,
 
 // "Longest extension" end point: both ends, end of query, end of database,
 // else the best cell of the last live anti-diagonal.
-__device__ inline void item_end(const Item &s, const Rings &A, int32_t &score,
-                                int &col, int &row, bool &upd)
+template <class T>
+__device__ inline void item_end(const Item &s, const Rings<T> &A,
+                                int32_t &score, int &col, int &row, bool &upd)
 {
     const int d = s.d;
     int lcol = s.l3 + s.off3 - 2;
     int lrow = d - lcol;
-    int32_t ls = A.at(s.r3, lcol);
+    int32_t ls = A.get(s.r3, lcol, s.f3);
     if (ls == kUndef) {
-        if (A.at(s.r2, s.l2 + s.off2 - 2) != kUndef) {
+        if (A.get(s.r2, s.l2 + s.off2 - 2, s.f2) != kUndef) {
             lcol = s.l2 + s.off2 - 2;
             lrow = d - 1 - lcol;
-            ls = A.at(s.r2, lcol);
-        } else if (s.l2 > 2 && A.at(s.r2, s.l2 + s.off2 - 3) != kUndef) {
+            ls = A.get(s.r2, lcol, s.f2);
+        } else if (s.l2 > 2 &&
+                   A.get(s.r2, s.l2 + s.off2 - 3, s.f2) != kUndef) {
             lcol = s.l2 + s.off2 - 3;
             lrow = d - 1 - lcol;
-            ls = A.at(s.r2, lcol);
+            ls = A.get(s.r2, lcol, s.f2);
         }
     }
     if (ls == kUndef) {
         for (int i = 0; i < s.l1; ++i) {
-            const int32_t v = A.at(s.r1, s.off1 + i);
+            const int32_t v = A.get(s.r1, s.off1 + i, s.f1);
             if (v > ls) {
                 ls = v;
                 lcol = i + s.off1;
@@ -303,26 +356,32 @@ constexpr int kEndBatch = 16;
 template <int RING>
 __global__ __launch_bounds__(RING ? 64 : 256) void k_xdrop_pl(XdArgs a)
 {
+    using T = typename std::conditional<(RING > 0), int16_t, int32_t>::type;
     __shared__ int8_t mat[32 * 32];
-    __shared__ int32_t ring[RING ? 3 * RING * 64 : 1];
-    __shared__ uint8_t res[RING ? 2 * 32 * 64 : 1];
+    __shared__ T ring[RING ? 3 * RING * 64 : 1];
+    __shared__ uint8_t res[RING ? 2 * RING * 64 : 1];
     for (int i = threadIdx.x; i < 32 * 32; i += blockDim.x)
         mat[i] = a.mat[i];
     __syncthreads();
     const uint32_t n = *a.count;
     const int lane = threadIdx.x & 63;
-    Rings A;
+    Rings<T> A;
     if (RING) {
-        A = Rings{ring + lane, RING * 64, 64, RING - 1, RING};
+        A = Rings<T>{ring + lane, RING * 64, 64, RING - 1, RING};
     } else {
-        int32_t *buf = a.scratch + (size_t)(blockIdx.x * blockDim.x +
-                                            threadIdx.x) *
-                                       (size_t)(3 * a.cap);
-        A = Rings{buf, a.cap, 1, a.cap - 1, a.cap};
+        // (T is int32_t here)
+        T *buf = (T *)a.scratch + (size_t)(blockIdx.x * blockDim.x +
+                                           threadIdx.x) *
+                                      (size_t)(3 * a.cap);
+        A = Rings<T>{buf, a.cap, 1, a.cap - 1, a.cap};
     }
-    const ResRings Rr{res + lane, res + 32 * 64 + lane};
+    const ResRings Rr{res + lane, res + RING * 64 + lane, RING - 1};
     constexpr bool ST = RING > 0;
     const int gap = a.gap, X = a.X;
+    // parameters the int16 cell encoding does not cover: every item
+    // overflows to the int32 kernels
+    const bool fits = !A.kRel || (X >= 0 && X <= kX16Max &&
+                                  gap >= kGap16Min && gap <= 0);
     int state = kIdle;
     uint32_t k = 0;
     Item s{};
@@ -391,7 +450,7 @@ __global__ __launch_bounds__(RING ? 64 : 256) void k_xdrop_pl(XdArgs a)
         if (!__ballot(state != kExhausted))
             break;
         if (state == kRun) {
-            const int st = item_step<ST>(s, A, Rr, mat, gap, X);
+            const int st = fits ? item_step<ST>(s, A, Rr, mat, gap, X) : 2;
             if (st == 1) {
                 state = kEnd;
             } else if (st == 2) {

@@ -59,6 +59,33 @@ def test_xdrop_extents_synthetic(model):
     check_extents(al, w.codes, w.offsets, w.pairs, model)
 
 
+# The per-lane kernel keeps cells as int16 offsets from each anti-diagonal's
+# X-drop limit for X <= 30000 and gaps in [-1000, -1]; beyond either bound
+# every item runs in the int32 kernels.  Extents on both sides of each bound.
+@pytest.mark.parametrize("xdrop,gap", [(30000, -11), (30001, -11),
+                                       (49, -1000), (49, -1001), (1, -1)])
+def test_xdrop_extents_at_the_ring_encoding_limits(xdrop, gap):
+    w = small_config2(n_seq=40, n_pairs=80, seed=17)
+    cfg = config_for_cli(-11, -2, 6, xdrop, 2, "as_called")
+    cfg.xdrop_gap = gap   # the DP gaps stay in their own range
+    al = Aligner(cfg)
+    al.load_seqs(w.codes, w.offsets)
+    codes, offs, pairs = w.codes, w.offsets, w.pairs
+    for s in range(2):
+        got = al.xdrop_extents(pairs, s)
+        for k in range(len(pairs)):
+            r, c = int(pairs["r"][k]), int(pairs["c"][k])
+            v = codes[offs[r]:offs[r + 1]]
+            h = codes[offs[c]:offs[c + 1]]
+            ro = int(pairs["seed"][k][2 * s])
+            co = int(pairs["seed"][k][2 * s + 1])
+            x = orc.xdrop_extend(h, v, co, ro, 6, gap, xdrop)
+            assert (got["r_beg"][k], got["r_end"][k], got["c_beg"][k],
+                    got["c_end"][k]) == (x["beg_v"], x["end_v"], x["beg_h"],
+                                         x["end_h"]), (xdrop, gap, k, s)
+    al.close()
+
+
 @pytest.mark.parametrize("model", ["as_called", "as_documented"])
 def test_sxa_operator_synthetic(model):
     w = small_config2()
