@@ -12,16 +12,19 @@
 // by absolute column (slot = column mod ring size).  The algorithm only reads
 // columns inside the range written on that anti-diagonal ([off, maxCol]), so
 // the ring is exact while that range fits; a wider range reports overflow.
-//   - k_xdrop_pl<RING>: RING-slot rings in LDS ([3][RING][64 lanes] int16,
-//     RING = 16 or 32).  A cell of anti-diagonal d is kept only if it is at
-//     least lim_d = best - X (best before d), and no cell exceeds best plus
-//     the largest matrix score, so a ring holds v - lim_d, in [0, X + 128]
-//     for int8 scores and gap <= 0, and -32768 for an undefined cell.  Each
-//     anti-diagonal keeps its own lim_d (its "frame"); the cell arithmetic
-//     runs in the frame of anti-diagonal d-1.  Items of calls with X above
-//     30000 or gaps outside [-1000, 0] overflow at once to the int32 kernels;
-//   - k_xdrop_pl<0>: int32 rings in global scratch sized to the longest
-//     sequence (the overflow path), frames all zero.
+//   - k_xdrop_pl<RING, T>: RING-slot rings in LDS ([3][RING][64 lanes] of
+//     int8 or int16 T, RING = 16 or 32).  A cell of anti-diagonal d is kept
+//     only if it is at least lim_d = best - X (best before d), and no cell
+//     exceeds best plus the largest matrix score (<= 31, as pa_create
+//     requires), so a ring holds v - lim_d - bias, in [-bias, X + 31 -
+//     bias], and the type's minimum for an undefined cell; bias is 64 for
+//     int8 (X <= 160, gap >= -32) and 0 for int16 (X <= 30000, gap >=
+//     -1000).  Each anti-diagonal keeps its own lim_d + bias (its "frame");
+//     the cell arithmetic runs in the frame of anti-diagonal d-1.  Items of
+//     calls outside the type's X and gap bounds overflow at once to the
+//     int32 kernels;
+//   - k_xdrop_pl<0, int32_t>: int32 rings in global scratch sized to the
+//     longest sequence (the overflow path), frames all zero.
 // Scheduling: each lane advances its own item one anti-diagonal per loop
 // trip and takes the next item from the queue as soon as its item ends (one
 // atomic per wave for all lanes that need an item), so a long extension
@@ -36,9 +39,6 @@ namespace pa {
 namespace {
 
 constexpr int32_t kUndef = INT_MIN / 4;
-// int16 rings: the X-drop and gap values the cell encoding covers
-constexpr int kX16Max = 30000;
-constexpr int kGap16Min = -1000;
 
 template <class T> struct Rings {
     T *base;
@@ -47,9 +47,20 @@ template <class T> struct Rings {
     int mask;          // slots - 1
     int slots;
     // stored value of an undefined cell
-    static constexpr int32_t kSent = sizeof(T) == 2 ? INT16_MIN : kUndef;
-    // values are kept relative to their anti-diagonal's frame
-    static constexpr bool kRel = sizeof(T) == 2;
+    static constexpr int32_t kSent =
+        sizeof(T) == 1 ? INT8_MIN : sizeof(T) == 2 ? INT16_MIN : kUndef;
+    // values are kept relative to their anti-diagonal's frame, lim_d + kBias
+    static constexpr bool kRel = sizeof(T) < 4;
+    static constexpr int32_t kBias = sizeof(T) == 1 ? 64 : 0;
+    // the X-drop and gap values the encoding covers: defined cells in
+    // [-kBias, X + 31 - kBias], and an undefined cell, after the shifts of
+    // the cell loop, below every defined candidate and below the limit
+    static constexpr int kXMax = sizeof(T) == 1 ? 160 : 30000;
+    static constexpr int kGapMin = sizeof(T) == 1 ? -32 : -1000;
+    static __host__ __device__ inline bool covers(int X, int gap)
+    {
+        return !kRel || (X >= 0 && X <= kXMax && gap >= kGapMin && gap <= 0);
+    }
     __device__ inline T &at(int r, int col) const
     {
         return base[r * ring_stride + (col & mask) * slot_stride];
@@ -142,7 +153,7 @@ __device__ inline void item_init(Item &s, const Rings<T> &A,
     s.l3 = 2;
     // anti-diagonals 0 and 1 in the frame of the first limit (best 0):
     // frames never decrease
-    const int32_t f0 = A.kRel ? -X : 0;
+    const int32_t f0 = A.kRel ? -X + A.kBias : 0;
     s.f1 = s.f2 = s.f3 = f0;
     A.at(s.r2, 0) = 0 - f0;
     A.at(s.r3, 0) = -gap > X ? A.kSent : gap - f0;
@@ -188,7 +199,7 @@ __device__ inline int item_step(Item &s, const Rings<T> &A,
     const int32_t lim = s.best - X;
     s.f1 = s.f2;
     s.f2 = s.f3;
-    s.f3 = A.kRel ? lim : 0;
+    s.f3 = A.kRel ? lim + A.kBias : 0;
     if (s.l3 > A.slots)
         return 2;
     const int r1 = s.r1, r2 = s.r2, r3 = s.r3;
@@ -353,35 +364,34 @@ constexpr uint32_t kChunk = 64;   // items claimed per queue atomic
 // claims then run once for all of them instead of in most steps for one.
 constexpr int kEndBatch = 16;
 
-template <int RING>
+template <int RING, class Cell>
 __global__ __launch_bounds__(RING ? 64 : 256) void k_xdrop_pl(XdArgs a)
 {
-    using T = typename std::conditional<(RING > 0), int16_t, int32_t>::type;
+    static_assert((RING > 0) == (sizeof(Cell) < 4), "LDS rings are narrow");
     __shared__ int8_t mat[32 * 32];
-    __shared__ T ring[RING ? 3 * RING * 64 : 1];
+    __shared__ Cell ring[RING ? 3 * RING * 64 : 1];
     __shared__ uint8_t res[RING ? 2 * RING * 64 : 1];
     for (int i = threadIdx.x; i < 32 * 32; i += blockDim.x)
         mat[i] = a.mat[i];
     __syncthreads();
     const uint32_t n = *a.count;
     const int lane = threadIdx.x & 63;
-    Rings<T> A;
+    Rings<Cell> A;
     if (RING) {
-        A = Rings<T>{ring + lane, RING * 64, 64, RING - 1, RING};
+        A = Rings<Cell>{ring + lane, RING * 64, 64, RING - 1, RING};
     } else {
-        // (T is int32_t here)
-        T *buf = (T *)a.scratch + (size_t)(blockIdx.x * blockDim.x +
+        // (Cell is int32_t here)
+        Cell *buf = (Cell *)a.scratch + (size_t)(blockIdx.x * blockDim.x +
                                            threadIdx.x) *
                                       (size_t)(3 * a.cap);
-        A = Rings<T>{buf, a.cap, 1, a.cap - 1, a.cap};
+        A = Rings<Cell>{buf, a.cap, 1, a.cap - 1, a.cap};
     }
     const ResRings Rr{res + lane, res + RING * 64 + lane, RING - 1};
     constexpr bool ST = RING > 0;
     const int gap = a.gap, X = a.X;
-    // parameters the int16 cell encoding does not cover: every item
-    // overflows to the int32 kernels
-    const bool fits = !A.kRel || (X >= 0 && X <= kX16Max &&
-                                  gap >= kGap16Min && gap <= 0);
+    // parameters the narrow cell encoding does not cover (the host picks
+    // the variant that does): every item overflows to the int32 kernels
+    const bool fits = Rings<Cell>::covers(X, gap);
     int state = kIdle;
     uint32_t k = 0;
     Item s{};
@@ -478,6 +488,16 @@ __global__ __launch_bounds__(RING ? 64 : 256) void k_xdrop_pl(XdArgs a)
 
 }  // namespace
 
+// The LDS-ring variant for `ring` slots: int8 cells where the call's X-drop
+// value and gap allow them, else int16.
+template <class F> static auto pick_pl(int ring, int xdrop, int gap, F &&f)
+{
+    const bool narrow = Rings<int8_t>::covers(xdrop, gap);
+    if (ring == 16)
+        return narrow ? f(k_xdrop_pl<16, int8_t>) : f(k_xdrop_pl<16, int16_t>);
+    return narrow ? f(k_xdrop_pl<32, int8_t>) : f(k_xdrop_pl<32, int16_t>);
+}
+
 hipError_t launch_xdrop(const XdTask *tasks, const uint32_t *list,
                         const uint32_t *count, uint32_t *queue,
                         const int8_t *mat, int gap, int xdrop, int seed_len,
@@ -487,29 +507,34 @@ hipError_t launch_xdrop(const XdTask *tasks, const uint32_t *list,
 {
     XdArgs a{tasks, list, count, queue, mat, gap, xdrop, seed_len,
              scratch, cap, out, ovf_list, ovf_count};
-    if (ring == 0)
-        hipLaunchKernelGGL(k_xdrop_pl<0>, dim3(grid), dim3(256), 0, s, a);
-    else if (ring == 16)
-        hipLaunchKernelGGL(k_xdrop_pl<16>, dim3(grid), dim3(64), 0, s, a);
-    else if (ring == 32)
-        hipLaunchKernelGGL(k_xdrop_pl<32>, dim3(grid), dim3(64), 0, s, a);
-    else
+    if (ring == 0) {
+        hipLaunchKernelGGL((k_xdrop_pl<0, int32_t>), dim3(grid), dim3(256), 0,
+                           s, a);
+    } else if (ring == 16 || ring == 32) {
+        pick_pl(ring, xdrop, gap, [&](auto k) {
+            hipLaunchKernelGGL(k, dim3(grid), dim3(64), 0, s, a);
+            return 0;
+        });
+    } else {
         return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
-int xdrop_occupancy(int ring)
+int xdrop_occupancy(int ring, int xdrop, int gap)
 {
-    static int cache[2] = {0, 0};
-    const int i = ring == 16 ? 0 : 1;
+    // per variant: ring 16 / 32, int8 / int16
+    static int cache[4] = {0, 0, 0, 0};
+    const int i = (ring == 16 ? 0 : 2) +
+                  (Rings<int8_t>::covers(xdrop, gap) ? 0 : 1);
     if (!cache[i]) {
-        int n = 0;
-        if (ring == 16)
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_xdrop_pl<16>,
-                                                         64, 0);
-        else
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_xdrop_pl<32>,
-                                                         64, 0);
+        const int n = pick_pl(ring, xdrop, gap, [](auto k) {
+            int m = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&m, k, 64, 0) !=
+                hipSuccess)
+                m = 0;
+            return m;
+        });
         cache[i] = n > 0 ? n : 1;
     }
     return cache[i];

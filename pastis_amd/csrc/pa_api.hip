@@ -209,7 +209,8 @@ int dp_grid(pa_ctx *c, uint64_t ntasks, int cls, bool local, bool trace,
 
 int xd_grid(pa_ctx *c, uint64_t nitems)
 {
-    int per_cu = xdrop_occupancy(c->tune.xd_ring);
+    int per_cu = xdrop_occupancy(c->tune.xd_ring, c->cfg.xdrop,
+                                 c->cfg.xdrop_gap);
     if (c->tune.xd_wg_per_cu > 0)
         per_cu = std::min(per_cu, (int)c->tune.xd_wg_per_cu);
     per_cu = std::min(per_cu, kMaxWgPerCu);

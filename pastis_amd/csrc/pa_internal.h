@@ -300,8 +300,9 @@ int xdrop_wave_occupancy(int wide);
 // Workgroups of one DP kernel variant (of a kernel shape) that fit on one
 // CU (occupancy).
 int dp_occupancy(int shape, bool local, bool trace, bool i16);
-// Same for the LDS-ring X-drop kernel with `ring` slots per lane.
-int xdrop_occupancy(int ring);
+// Same for the LDS-ring X-drop kernel with `ring` slots per lane (the
+// variant launch_xdrop picks for these X-drop and gap values).
+int xdrop_occupancy(int ring, int xdrop, int gap);
 
 hipError_t launch_xdrop(const XdTask *tasks, const uint32_t *list,
                         const uint32_t *count, uint32_t *queue,

@@ -59,10 +59,12 @@ def test_xdrop_extents_synthetic(model):
     check_extents(al, w.codes, w.offsets, w.pairs, model)
 
 
-# The per-lane kernel keeps cells as int16 offsets from each anti-diagonal's
-# X-drop limit for X <= 30000 and gaps in [-1000, -1]; beyond either bound
-# every item runs in the int32 kernels.  Extents on both sides of each bound.
-@pytest.mark.parametrize("xdrop,gap", [(30000, -11), (30001, -11),
+# The per-lane kernel keeps cells as int8 offsets from each anti-diagonal's
+# X-drop limit for X <= 160 and gaps in [-32, -1], as int16 offsets for X <=
+# 30000 and gaps in [-1000, -1]; beyond either bound every item runs in the
+# int32 kernels.  Extents on both sides of each bound.
+@pytest.mark.parametrize("xdrop,gap", [(160, -11), (161, -11), (49, -32),
+                                       (49, -33), (30000, -11), (30001, -11),
                                        (49, -1000), (49, -1001), (1, -1)])
 def test_xdrop_extents_at_the_ring_encoding_limits(xdrop, gap):
     w = small_config2(n_seq=40, n_pairs=80, seed=17)
