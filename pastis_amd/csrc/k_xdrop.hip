@@ -240,12 +240,11 @@ __device__ inline int item_step(Item &s, const Rings<T> &A,
         const int dr = STAGE ? (int)Rr.ds(d - col) : (int)s.ds[dp];
         const int32_t dg = A.at(r1, col - 1) + c12 + mat[qr * 32 + dr];
         tmp = dg > tmp ? dg : tmp;
-        if (tmp < lim2) {
-            A.at(r3, col) = A.kSent;
-        } else {
-            A.at(r3, col) = tmp - st2;
-            adbest = tmp > adbest ? tmp : adbest;
-        }
+        // kept (>= the limit) or undefined, as selects: no branch per cell
+        const bool keep = tmp >= lim2;
+        A.at(r3, col) = keep ? tmp - st2 : A.kSent;
+        const int32_t kv = keep ? tmp : adbest;
+        adbest = kv > adbest ? kv : adbest;
         l = u;
     }
     adbest += s.f2;
