@@ -127,6 +127,12 @@ template <> struct Ops<true> {
     {
         typedef unsigned short u2v __attribute__((ext_vector_type(2)));
         const V B = splat(base);
+        // the multiplier as an opaque register value: n * MUL + base is then
+        // one packed multiply-add (v_pk_mad_u16) instead of a shift and an add
+        int mul = MUL;
+        if (MUL != 1)
+            asm volatile("" : "+s"(mul));
+        const V M = splat(mul);
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             // [byte k of lw, 0, byte k of hw, 0]
@@ -136,8 +142,8 @@ template <> struct Ops<true> {
             const V n0 = v(p & 0x000f000fu);
             const V n1 = __builtin_bit_cast(
                 V, __builtin_bit_cast(u2v, p) >> (unsigned short)4);
-            s[k] = MUL == 1 ? add(n0, B) : n0 * splat(MUL) + B;
-            s[k + 4] = MUL == 1 ? add(n1, B) : n1 * splat(MUL) + B;
+            s[k] = MUL == 1 ? add(n0, B) : n0 * M + B;
+            s[k + 4] = MUL == 1 ? add(n1, B) : n1 * M + B;
         }
     }
     // new row-above input: lo <- hi of lane-1, hi <- own lo (lane_up).
