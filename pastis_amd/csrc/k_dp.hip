@@ -131,7 +131,7 @@ template <> struct Ops<true> {
         // one packed multiply-add (v_pk_mad_u16) instead of a shift and an add
         int mul = MUL;
         if (MUL != 1)
-            asm volatile("" : "+s"(mul));
+            PA_OPAQUE(mul);
         const V M = splat(mul);
 #pragma unroll
         for (int k = 0; k < 4; ++k) {

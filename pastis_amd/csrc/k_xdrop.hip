@@ -79,6 +79,7 @@ struct Item {
     int cols, rows;           // segment lengths + 1
     bool right;
     int d, minCol, maxCol, best;
+    int dgap;                 // d * gap
     int off1, off2, off3, l1, l2, l3;
     int r1, r2, r3;           // ring of anti-diagonals d-2, d-1, d
     int32_t f1, f2, f3;       // their frames (stored value = value - frame)
@@ -94,11 +95,11 @@ struct Item {
 // Residues of column col (query) and row row (database) of the item.
 __device__ inline int q_at(const Item &s, int col)
 {
-    return s.qb[s.qstep * col];
+    return s.qb[s.qstep > 0 ? col : -col];   // qstep is +1 or -1
 }
 __device__ inline int d_at(const Item &s, int row)
 {
-    return s.db[s.dstep * row];
+    return s.db[s.dstep > 0 ? row : -row];
 }
 
 // Per-lane LDS rings of residues, as many slots as the cell rings (RING),
@@ -162,6 +163,7 @@ __device__ inline void item_init(Item &s, const Rings<T> &A,
     s.maxCol = 2;
     s.off1 = s.off2 = s.off3 = 0;
     s.d = 1;
+    s.dgap = gap;
     s.best = 0;
     s.nc = 0;
     if (STAGE) {
@@ -207,11 +209,14 @@ __device__ inline int item_step(Item &s, const Rings<T> &A,
     // _initAntiDiag3: edge entries, first column / first row cells
     A.at(r3, off3) = A.kSent;
     A.at(r3, maxCol0) = A.kSent;
-    if ((long)d * gap > (long)s.best - X) {
+    // d * gap as a running sum; with narrow cells (|gap| <= 1000, X <=
+    // 30000) it cannot overflow int32, else the comparison is made in 64 bits
+    const int32_t dgap = (s.dgap += gap);
+    if (A.kRel ? dgap > s.best - X : (long)d * gap > (long)s.best - X) {
         if (off3 == 0)
-            A.at(r3, off3) = d * gap - s.f3;
+            A.at(r3, off3) = dgap - s.f3;
         if (d - maxCol0 == 0)
-            A.at(r3, maxCol0) = d * gap - s.f3;
+            A.at(r3, maxCol0) = dgap - s.f3;
     }
     const int cols = s.cols, rows = s.rows;
     // the cells are computed in the frame of anti-diagonal d-1 (r2): c12
@@ -221,7 +226,7 @@ __device__ inline int item_step(Item &s, const Rings<T> &A,
     const int32_t c12 = s.f1 - s.f2;
     const int32_t lim2 = lim - s.f2;
     const int32_t st2 = s.f3 - s.f2;
-    int32_t adbest = d * gap - s.f2;
+    int32_t adbest = dgap - s.f2;
     // the up value of one cell is the left value of the next (r2 is not
     // written in this loop)
     int32_t l = A.at(r2, s.minCol - 1);

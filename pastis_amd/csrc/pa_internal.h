@@ -21,6 +21,11 @@ namespace pa {
 #ifndef PA_WAVES_PER_EU   // the CPU emulation build of the tests: empty
 #define PA_WAVES_PER_EU(n) __attribute__((amdgpu_waves_per_eu(n)))
 #endif
+// Makes a wave-uniform integer opaque to the optimiser (no instruction): it
+// is then kept in a scalar register and not folded into the code using it.
+#ifndef PA_OPAQUE   // the CPU emulation build: a general register
+#define PA_OPAQUE(x) asm volatile("" : "+s"(x))
+#endif
 template <class T> __device__ inline T PA_GLOBAL *glb(T *p)
 {
     return (T PA_GLOBAL *)p;

@@ -28,6 +28,7 @@
 #define __constant__ static const
 #define PA_GLOBAL   // host memory has one address space
 #define PA_WAVES_PER_EU(n)   // no register allocation to constrain
+#define PA_OPAQUE(x) asm volatile("" : "+r"(x))
 #define __forceinline__ inline
 #define HIP_DYNAMIC_SHARED(type, var) type *var = (type *)emu::dyn_shared();
 #define __align__(n) __attribute__((aligned(n)))
