@@ -8,10 +8,11 @@
 // extension" end point follow the oracle's or_xdrop_one step for step, so
 // extents and evaluated-cell counts are bit-identical.
 //
-// Storage of the three live anti-diagonals: a ring per anti-diagonal indexed
-// by absolute column (slot = column mod ring size).  The algorithm only reads
-// columns inside the range written on that anti-diagonal ([off, maxCol]), so
-// the ring is exact while that range fits; a wider range reports overflow.
+// Storage of the three live anti-diagonals: one array of `slots` cells per
+// anti-diagonal, column c of an anti-diagonal whose range starts at column
+// off in slot c - off.  The algorithm only reads columns inside the range
+// written on that anti-diagonal ([off, maxCol]), so the arrays are exact
+// while that range fits; a wider range reports overflow.
 //   - k_xdrop_pl<RING, T>: RING-slot rings in LDS ([3][RING][64 lanes] of
 //     int8 or int16 T, RING = 16 or 32).  A cell of anti-diagonal d is kept
 //     only if it is at least lim_d = best - X (best before d), and no cell
@@ -42,9 +43,8 @@ constexpr int32_t kUndef = INT_MIN / 4;
 
 template <class T> struct Rings {
     T *base;
-    int ring_stride;   // elements between the three rings
+    int ring_stride;   // elements between the three anti-diagonals
     int slot_stride;   // elements between consecutive slots
-    int mask;          // slots - 1
     int slots;
     // stored value of an undefined cell
     static constexpr int32_t kSent =
@@ -61,14 +61,16 @@ template <class T> struct Rings {
     {
         return !kRel || (X >= 0 && X <= kXMax && gap >= kGapMin && gap <= 0);
     }
-    __device__ inline T &at(int r, int col) const
+    // column col of anti-diagonal array r, whose range starts at off
+    __device__ inline T &at(int r, int col, int off) const
     {
-        return base[r * ring_stride + (col & mask) * slot_stride];
+        return base[r * ring_stride + (col - off) * slot_stride];
     }
     // the value of a cell (kUndef if undefined) from its stored form
-    __device__ inline int32_t get(int r, int col, int32_t frame) const
+    __device__ inline int32_t get(int r, int col, int off,
+                                  int32_t frame) const
     {
-        const int32_t v = at(r, col);
+        const int32_t v = at(r, col, off);
         return v == kSent ? kUndef : v + frame;
     }
 };
@@ -156,9 +158,9 @@ __device__ inline void item_init(Item &s, const Rings<T> &A,
     // frames never decrease
     const int32_t f0 = A.kRel ? -X + A.kBias : 0;
     s.f1 = s.f2 = s.f3 = f0;
-    A.at(s.r2, 0) = 0 - f0;
-    A.at(s.r3, 0) = -gap > X ? A.kSent : gap - f0;
-    A.at(s.r3, 1) = -gap > X ? A.kSent : gap - f0;
+    A.at(s.r2, 0, 0) = 0 - f0;
+    A.at(s.r3, 0, 0) = -gap > X ? A.kSent : gap - f0;
+    A.at(s.r3, 1, 0) = -gap > X ? A.kSent : gap - f0;
     s.minCol = 1;
     s.maxCol = 2;
     s.off1 = s.off2 = s.off3 = 0;
@@ -207,16 +209,16 @@ __device__ inline int item_step(Item &s, const Rings<T> &A,
     const int r1 = s.r1, r2 = s.r2, r3 = s.r3;
     const int off3 = s.off3, maxCol0 = s.maxCol;
     // _initAntiDiag3: edge entries, first column / first row cells
-    A.at(r3, off3) = A.kSent;
-    A.at(r3, maxCol0) = A.kSent;
+    A.at(r3, off3, off3) = A.kSent;
+    A.at(r3, maxCol0, off3) = A.kSent;
     // d * gap as a running sum; with narrow cells (|gap| <= 1000, X <=
     // 30000) it cannot overflow int32, else the comparison is made in 64 bits
     const int32_t dgap = (s.dgap += gap);
     if (A.kRel ? dgap > s.best - X : (long)d * gap > (long)s.best - X) {
         if (off3 == 0)
-            A.at(r3, off3) = dgap - s.f3;
+            A.at(r3, off3, off3) = dgap - s.f3;
         if (d - maxCol0 == 0)
-            A.at(r3, maxCol0) = dgap - s.f3;
+            A.at(r3, maxCol0, off3) = dgap - s.f3;
     }
     const int cols = s.cols, rows = s.rows;
     // the cells are computed in the frame of anti-diagonal d-1 (r2): c12
@@ -229,7 +231,13 @@ __device__ inline int item_step(Item &s, const Rings<T> &A,
     int32_t adbest = dgap - s.f2;
     // the up value of one cell is the left value of the next (r2 is not
     // written in this loop)
-    int32_t l = A.at(r2, s.minCol - 1);
+    const int off1 = s.off1, off2 = s.off2;
+    int32_t l = A.at(r2, s.minCol - 1, off2);
+    // cell pointers of column minCol (r1: minCol - 1), stepped per column
+    const T *p1 = &A.at(r1, s.minCol - 1, off1);
+    const T *p2 = &A.at(r2, s.minCol, off2);
+    T *p3 = &A.at(r3, s.minCol, off3);
+    const int ss = A.slot_stride;
     for (int col = s.minCol; col < maxCol0; ++col) {
         int qp, dp;
         if (s.right) {
@@ -239,15 +247,18 @@ __device__ inline int item_step(Item &s, const Rings<T> &A,
             qp = cols - 1 - col;
             dp = rows - 1 + col - d;
         }
-        const int32_t u = A.at(r2, col);
+        const int32_t u = *p2;
         int32_t tmp = (l > u ? l : u) + gap;
         const int qr = STAGE ? (int)Rr.qs(col) : (int)s.qs[qp];
         const int dr = STAGE ? (int)Rr.ds(d - col) : (int)s.ds[dp];
-        const int32_t dg = A.at(r1, col - 1) + c12 + mat[qr * 32 + dr];
+        const int32_t dg = *p1 + c12 + mat[qr * 32 + dr];
         tmp = dg > tmp ? dg : tmp;
         // kept (>= the limit) or undefined, as selects: no branch per cell
         const bool keep = tmp >= lim2;
-        A.at(r3, col) = keep ? tmp - st2 : A.kSent;
+        *p3 = keep ? tmp - st2 : A.kSent;
+        p1 += ss;
+        p2 += ss;
+        p3 += ss;
         const int32_t kv = keep ? tmp : adbest;
         adbest = kv > adbest ? kv : adbest;
         l = u;
@@ -256,11 +267,12 @@ __device__ inline int item_step(Item &s, const Rings<T> &A,
     s.nc += (uint32_t)(maxCol0 - s.minCol);
     s.best = adbest > s.best ? adbest : s.best;
     int minCol = s.minCol, maxCol = maxCol0;
-    while (minCol - off3 < s.l3 && A.at(r3, minCol) == A.kSent &&
-           minCol - s.off2 - 1 < s.l2 && A.at(r2, minCol - 1) == A.kSent)
+    while (minCol - off3 < s.l3 && A.at(r3, minCol, off3) == A.kSent &&
+           minCol - off2 - 1 < s.l2 &&
+           A.at(r2, minCol - 1, off2) == A.kSent)
         ++minCol;
-    while (maxCol - off3 > 0 && A.at(r3, maxCol - 1) == A.kSent &&
-           A.at(r2, maxCol - 1) == A.kSent)
+    while (maxCol - off3 > 0 && A.at(r3, maxCol - 1, off3) == A.kSent &&
+           A.at(r2, maxCol - 1, off2) == A.kSent)
         --maxCol;
     ++maxCol;
     if (minCol < d + 2 - rows)
@@ -281,22 +293,22 @@ __device__ inline void item_end(const Item &s, const Rings<T> &A,
     const int d = s.d;
     int lcol = s.l3 + s.off3 - 2;
     int lrow = d - lcol;
-    int32_t ls = A.get(s.r3, lcol, s.f3);
+    int32_t ls = A.get(s.r3, lcol, s.off3, s.f3);
     if (ls == kUndef) {
-        if (A.get(s.r2, s.l2 + s.off2 - 2, s.f2) != kUndef) {
+        if (A.get(s.r2, s.l2 + s.off2 - 2, s.off2, s.f2) != kUndef) {
             lcol = s.l2 + s.off2 - 2;
             lrow = d - 1 - lcol;
-            ls = A.get(s.r2, lcol, s.f2);
+            ls = A.get(s.r2, lcol, s.off2, s.f2);
         } else if (s.l2 > 2 &&
-                   A.get(s.r2, s.l2 + s.off2 - 3, s.f2) != kUndef) {
+                   A.get(s.r2, s.l2 + s.off2 - 3, s.off2, s.f2) != kUndef) {
             lcol = s.l2 + s.off2 - 3;
             lrow = d - 1 - lcol;
-            ls = A.get(s.r2, lcol, s.f2);
+            ls = A.get(s.r2, lcol, s.off2, s.f2);
         }
     }
     if (ls == kUndef) {
         for (int i = 0; i < s.l1; ++i) {
-            const int32_t v = A.get(s.r1, s.off1 + i, s.f1);
+            const int32_t v = A.get(s.r1, s.off1 + i, s.off1, s.f1);
             if (v > ls) {
                 ls = v;
                 lcol = i + s.off1;
@@ -368,8 +380,11 @@ constexpr uint32_t kChunk = 64;   // items claimed per queue atomic
 // claims then run once for all of them instead of in most steps for one.
 constexpr int kEndBatch = 16;
 
+// the LDS variants: at most 80 VGPRs, six waves per SIMD (their LDS, 6 KB
+// per wave with int8 cells, allows more)
 template <int RING, class Cell>
-__global__ __launch_bounds__(RING ? 64 : 256) void k_xdrop_pl(XdArgs a)
+__global__ __launch_bounds__(RING ? 64 : 256)
+PA_WAVES_PER_EU(RING ? 6 : 1) void k_xdrop_pl(XdArgs a)
 {
     static_assert((RING > 0) == (sizeof(Cell) < 4), "LDS rings are narrow");
     __shared__ int8_t mat[32 * 32];
@@ -382,13 +397,13 @@ __global__ __launch_bounds__(RING ? 64 : 256) void k_xdrop_pl(XdArgs a)
     const int lane = threadIdx.x & 63;
     Rings<Cell> A;
     if (RING) {
-        A = Rings<Cell>{ring + lane, RING * 64, 64, RING - 1, RING};
+        A = Rings<Cell>{ring + lane, RING * 64, 64, RING};
     } else {
         // (Cell is int32_t here)
         Cell *buf = (Cell *)a.scratch + (size_t)(blockIdx.x * blockDim.x +
                                            threadIdx.x) *
                                       (size_t)(3 * a.cap);
-        A = Rings<Cell>{buf, a.cap, 1, a.cap - 1, a.cap};
+        A = Rings<Cell>{buf, a.cap, 1, a.cap};
     }
     const ResRings Rr{res + lane, res + RING * 64 + lane, RING - 1};
     constexpr bool ST = RING > 0;
