@@ -907,7 +907,6 @@ namespace {
 // step back one record inside the run, vertical moves stay in the record,
 // so a run serves several steps.  Without STAGE each step loads its byte
 // from global memory.
-constexpr int kTbThreads = 256;
 
 // Residues of one sequence as the walk reads them (backwards, mostly one
 // position per diagonal step): the aligned 8-byte word holding the last one

@@ -471,7 +471,9 @@ int run_dp(pa_ctx *c, uint64_t nt, bool local, bool trace, int kind,
                 tb.count = ctr + C_BIN + bin;
                 HIPCHK(c, launch_traceback(tb, c->shapes.shape[cls],
                                            c->tune.tb_stage != 0,
-                                           (int)((b - a + 255) / 256), ts));
+                                           (int)((b - a + kTbThreads - 1) /
+                                                 kTbThreads),
+                                           ts));
                 c->stats.launches++;
             }
             return PA_OK;

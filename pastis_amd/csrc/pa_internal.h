@@ -213,6 +213,9 @@ __host__ __device__ inline int lanes_per_task(uint32_t qlen, const DpShapes &sh)
 // cache lines.  A pass is padded to whole segments.
 constexpr uint32_t kTraceSeg = 8;
 
+// Threads (one traceback walk each) per workgroup of the traceback kernels.
+constexpr int kTbThreads = 256;
+
 __host__ __device__ inline uint32_t trace_last_lanes(uint32_t qlen, int R,
                                                      int L)
 {
