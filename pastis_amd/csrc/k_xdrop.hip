@@ -13,18 +13,18 @@
 // off in slot c - off.  The algorithm only reads columns inside the range
 // written on that anti-diagonal ([off, maxCol]), so the arrays are exact
 // while that range fits; a wider range reports overflow.
-//   - k_xdrop_pl<RING, T>: RING-slot rings in LDS ([3][RING][64 lanes] of
+//   - k_xdrop_pl<RING, T>: RING-slot arrays in LDS ([3][RING][64 lanes] of
 //     int8 or int16 T, RING = 16 or 32).  A cell of anti-diagonal d is kept
 //     only if it is at least lim_d = best - X (best before d), and no cell
 //     exceeds best plus the largest matrix score (<= 31, as pa_create
-//     requires), so a ring holds v - lim_d - bias, in [-bias, X + 31 -
+//     requires), so an array holds v - lim_d - bias, in [-bias, X + 31 -
 //     bias], and the type's minimum for an undefined cell; bias is 64 for
 //     int8 (X <= 160, gap >= -32) and 0 for int16 (X <= 30000, gap >=
 //     -1000).  Each anti-diagonal keeps its own lim_d + bias (its "frame");
 //     the cell arithmetic runs in the frame of anti-diagonal d-1.  Items of
 //     calls outside the type's X and gap bounds overflow at once to the
 //     int32 kernels;
-//   - k_xdrop_pl<0, int32_t>: int32 rings in global scratch sized to the
+//   - k_xdrop_pl<0, int32_t>: int32 arrays in global scratch sized to the
 //     longest sequence (the overflow path), frames all zero.
 // Scheduling: each lane advances its own item one anti-diagonal per loop
 // trip and takes the next item from the queue as soon as its item ends (one
@@ -83,7 +83,7 @@ struct Item {
     int d, minCol, maxCol, best;
     int dgap;                 // d * gap
     int off1, off2, off3, l1, l2, l3;
-    int r1, r2, r3;           // ring of anti-diagonals d-2, d-1, d
+    int r1, r2, r3;           // arrays of anti-diagonals d-2, d-1, d
     int32_t f1, f2, f3;       // their frames (stored value = value - frame)
     uint32_t nc;              // evaluated cells
     // residue staging (LDS kernels): the request issued at the start of the
@@ -104,7 +104,7 @@ __device__ inline int d_at(const Item &s, int row)
     return s.db[s.dstep > 0 ? row : -row];
 }
 
-// Per-lane LDS rings of residues, as many slots as the cell rings (RING),
+// Per-lane LDS rings of residues, as many slots as the cell arrays (RING),
 // indexed by column and row.  An anti-diagonal that runs has maxCol - minCol
 // <= RING - 2, minCol never decreases, maxCol grows by at most one per step
 // and d - maxCol never decreases.  So the column stored at the start of step
@@ -179,7 +179,7 @@ __device__ inline void item_init(Item &s, const Rings<T> &A,
 }
 
 // One anti-diagonal.  Returns 0 while the band is not empty, 1 when the
-// extension ended, 2 when the written range outgrew the rings.
+// extension ended, 2 when the written range outgrew the arrays.
 template <bool STAGE, class T>
 __device__ inline int item_step(Item &s, const Rings<T> &A,
                                 const ResRings &Rr, const int8_t *mat,
@@ -329,7 +329,7 @@ struct XdArgs {
     uint32_t *queue;
     const int8_t *mat;
     int gap, X, seed_len;
-    int32_t *scratch;       // global-scratch rings (RING == 0)
+    int32_t *scratch;       // global-scratch arrays (RING == 0)
     int cap;                // their slots (power of two)
     XdOut *out;
     uint32_t *ovf_list, *ovf_count;
@@ -388,7 +388,7 @@ template <int RING, class Cell>
 __global__ __launch_bounds__(RING ? 64 : 256)
 PA_WAVES_PER_EU(RING ? 6 : 1) void k_xdrop_pl(XdArgs a)
 {
-    static_assert((RING > 0) == (sizeof(Cell) < 4), "LDS rings are narrow");
+    static_assert((RING > 0) == (sizeof(Cell) < 4), "LDS cells are narrow");
     __shared__ int8_t mat[32 * 32];
     __shared__ Cell ring[RING ? 3 * RING * 64 : 1];
     __shared__ uint8_t res[RING ? 2 * RING * 64 : 1];
@@ -495,7 +495,7 @@ PA_WAVES_PER_EU(RING ? 6 : 1) void k_xdrop_pl(XdArgs a)
         if (ending && (__popcll(ending) >= kEndBatch ||
                        !__ballot(state == kRun))) {
             if (state == kEnd) {
-                // the rings still hold the item's last anti-diagonals
+                // the arrays still hold the item's last anti-diagonals
                 int32_t score;
                 int col, row;
                 bool upd;
