@@ -64,7 +64,7 @@ def _align_eager(al, d_pairs, n, d_out):
     return al.d2h(out, d_out)
 
 
-def _align_replays(model, xdrop, w, trace_bytes=None):
+def _align_replays(model, xdrop, w, trace_bytes=None, tuning=None):
     """pa_align_graph over a batch whose contents change between calls,
     with eager pa_align_dev work in between; every call against eager."""
     import os
@@ -79,6 +79,8 @@ def _align_replays(model, xdrop, w, trace_bytes=None):
                 del os.environ["PA_TRACE_BYTES"]
             else:
                 os.environ["PA_TRACE_BYTES"] = old
+    if tuning:
+        al.set_tuning(**tuning)
     al.load_seqs(w.codes, w.offsets)
     n = len(w.pairs)
     d_pairs = al.device_alloc(w.pairs.nbytes)
@@ -122,6 +124,18 @@ def test_traced_graph_with_device_sized_chunks():
     _align_replays("as_called", 49, small_config2(n_seq=150, n_pairs=2000,
                                                   seed=54),
                    trace_bytes=16 << 20)
+
+
+@pytest.mark.parametrize("overlap", [1, 2])
+@pytest.mark.parametrize("trace_bytes", [None, 16 << 20])
+def test_traced_graph_with_walks_beside_the_dp(overlap, trace_bytes):
+    """Traceback walks on the second stream beside the DP (each class's
+    beside the next classes', or the shorter classes' beside the last
+    class's): the capture forks and joins that stream through events, in
+    one chunk and in many."""
+    _align_replays("as_called", 49, small_config2(n_seq=150, n_pairs=2000,
+                                                  seed=56),
+                   trace_bytes=trace_bytes, tuning=dict(tb_overlap=overlap))
 
 
 def test_traced_graph_overflow_is_flagged():
