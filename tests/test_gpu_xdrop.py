@@ -70,11 +70,9 @@ def test_xdrop_extents_at_the_ring_encoding_limits(xdrop, gap):
     w = small_config2(n_seq=40, n_pairs=80, seed=17)
     cfg = config_for_cli(-11, -2, 6, xdrop, 2, "as_called")
     cfg.xdrop_gap = gap   # the DP gaps stay in their own range
-    al = Aligner(cfg)
-    al.load_seqs(w.codes, w.offsets)
     codes, offs, pairs = w.codes, w.offsets, w.pairs
+    want = []
     for s in range(2):
-        got = al.xdrop_extents(pairs, s)
         for k in range(len(pairs)):
             r, c = int(pairs["r"][k]), int(pairs["c"][k])
             v = codes[offs[r]:offs[r + 1]]
@@ -82,10 +80,18 @@ def test_xdrop_extents_at_the_ring_encoding_limits(xdrop, gap):
             ro = int(pairs["seed"][k][2 * s])
             co = int(pairs["seed"][k][2 * s + 1])
             x = orc.xdrop_extend(h, v, co, ro, 6, gap, xdrop)
-            assert (got["r_beg"][k], got["r_end"][k], got["c_beg"][k],
-                    got["c_end"][k]) == (x["beg_v"], x["end_v"], x["beg_h"],
-                                         x["end_h"]), (xdrop, gap, k, s)
-    al.close()
+            want.append((x["beg_v"], x["end_v"], x["beg_h"], x["end_h"]))
+    for ring in (16, 32):   # both per-lane variants of each cell width
+        al = Aligner(cfg)
+        al.set_tuning(xd_ring=ring)
+        al.load_seqs(codes, offs)
+        for s in range(2):
+            got = al.xdrop_extents(pairs, s)
+            for k in range(len(pairs)):
+                assert (got["r_beg"][k], got["r_end"][k], got["c_beg"][k],
+                        got["c_end"][k]) == want[s * len(pairs) + k], \
+                    (xdrop, gap, ring, k, s)
+        al.close()
 
 
 @pytest.mark.parametrize("model", ["as_called", "as_documented"])
