@@ -128,6 +128,27 @@ def test_emulated_sxa_under_sanitizers(driver, tmp_path, coop, shapes, model,
     assert cells == int(ref["ext"]["cells"].sum())
 
 
+def test_emulated_xdrop_int16_cells(driver, tmp_path):
+    """X = 200 is beyond the int8 cell encoding of the per-lane X-drop
+    kernel (X <= 160): its int16 variant, and the band kernels for the items
+    that outgrow 16 columns, against the oracle under the sanitizers."""
+    w = small_config2(n_seq=20, n_pairs=16, seed=73)
+    pairs = w.pairs[:2]   # eight work items: the band kernels emulate slowly
+    got, ext, cells = run(driver, tmp_path, w, pairs, 1, xdrop=200)
+    ref = orc.sxa_batch(w.codes, w.offsets, pairs["r"], pairs["c"],
+                        pairs["seed"], oracle_params("as_called", xdrop=200))
+    for gf in ("score", "matches", "cols", "gap_opens"):
+        assert np.array_equal(got[gf].astype(np.int64),
+                              ref[gf].astype(np.int64)), gf
+    for s in range(2):
+        e = ref["ext"][:, s]
+        assert np.array_equal(ext[s]["r_beg"], e["beg_v"])
+        assert np.array_equal(ext[s]["r_end"], e["end_v"])
+        assert np.array_equal(ext[s]["c_beg"], e["beg_h"])
+        assert np.array_equal(ext[s]["c_end"], e["end_h"])
+    assert cells == int(ref["ext"]["cells"].sum())
+
+
 def run_kmer(driver, tmp_path, strs, k=3, stride=1, alph=0, sym=0, ckthr=0):
     inp = tmp_path / "k_in.bin"
     outp = tmp_path / "k_out.bin"
